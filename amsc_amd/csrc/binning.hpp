@@ -1,0 +1,66 @@
+// binning.hpp -- device binning of particles by cell and integer scans.
+#pragma once
+
+#include "common.hpp"
+
+namespace amsc {
+
+// Reusable device workspace for the 3-phase exclusive scan.
+struct ScanWork {
+  int *bsum = nullptr;  // per-block sums
+  int nb_cap = 0;
+};
+
+// out[0..n] = exclusive prefix of in[0..n), out[n] = total.  If zero_in,
+// in[] is reset to 0 after it is read.  If total_host != nullptr the total
+// is copied back (synchronising the stream).
+// The kernels do nothing when *active == 0 (active may be null).
+int exclusive_scan(const int *in, int *out, int n, bool zero_in,
+                   ScanWork &w, hipStream_t st, int *total_host,
+                   const int *active = nullptr);
+
+// Device binning pipeline state for one particle set.
+struct Binning {
+  int nrows = 0, ncols = 0;
+  int *count = nullptr;   // [nc] cell counts (kept zero between uses)
+  int *cstart = nullptr;  // [nc+1]
+  ScanWork scan;
+};
+
+int binning_alloc(Binning &b, int nrows, int ncols);
+void binning_free(Binning &b);
+
+// key[i] = cell of (x[i], y[i]); slot[i] = arrival rank in its cell.
+int launch_key_count(int np, const double *x, const double *y, int nrows,
+                     int ncols, double hx, double hy, int *key, int *slot,
+                     int *count, const int *active, hipStream_t st);
+// cstart from counts, then cptcls from (key, slot), then ascending particle
+// ids within every cell.
+int launch_finish_binning(Binning &b, int np, const int *key,
+                          const int *slot, int *cptcls, const int *active,
+                          hipStream_t st);
+// ccidx for the colouring colour = (r&1) + 2*(c&1); coloff on the host.
+int launch_colouring(int nrows, int ncols, int *ccidx, int *coloff,
+                     hipStream_t st);
+
+// Wave-aggregated atomic increment of count[key]: lanes of a wavefront with
+// equal keys share one atomic; the returned slot is the lane's rank among
+// them (ascending lane = ascending particle id) plus the base.
+__device__ __forceinline__ int wave_slot(int *count, int key, bool valid) {
+  unsigned long long todo = __ballot(valid);
+  int slot = 0;
+  const unsigned lane = __lane_id();
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lkey = __shfl(key, leader);
+    const unsigned long long peers = __ballot(valid && key == lkey) & todo;
+    int base = 0;
+    if ((int)lane == leader) base = atomicAdd(&count[lkey], __popcll(peers));
+    base = __shfl(base, leader);
+    if ((peers >> lane) & 1ull) slot = base + __popcll(peers & lanemask_lt());
+    todo &= ~peers;
+  }
+  return slot;
+}
+
+}  // namespace amsc

@@ -1,0 +1,120 @@
+// common.hpp -- shared device helpers and error plumbing for the gfx950
+// depth-averaged MPM library.
+//
+// Grid layout (quadgrid, as ported in main/gpu_kernels.cpp:7-31):
+//   node id  r + c*(nrows+1)   cell id  r + c*nrows
+//   local nodes 0:(c,r) 1:(c,r+1) 2:(c+1,r) 3:(c+1,r+1)
+// Bilinear basis with xi = (x - c*hx)/hx, eta = (y - r*hy)/hy, evaluated
+// in the reference's expression order (gpu_kernels.cpp:33-59), so that with
+// FMA contraction disabled every value is bit-identical to the reference's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "amsc_mpm.h"
+
+#include <cstdint>
+#include <string>
+
+namespace amsc {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+hipStream_t current_stream();
+void set_stream(hipStream_t s);
+const char *last_error_cstr();
+
+#define AMSC_HIP(call)                                                       \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return ::amsc::fail(AMSC_ERR_HIP, std::string(#call) + ": " +          \
+                                            hipGetErrorString(e_));          \
+  } while (0)
+
+#define AMSC_LAUNCH_CHECK()                                                  \
+  do {                                                                       \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess)                                                    \
+      return ::amsc::fail(AMSC_ERR_HIP, std::string("kernel launch: ") +     \
+                                            hipGetErrorString(e_));          \
+  } while (0)
+
+inline unsigned grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  return (unsigned)(g > 0 ? g : 1);
+}
+
+// ----------------------------------------------------------- geometry
+__device__ __forceinline__ int node_id(int inode, int c, int r, int nrows) {
+  const int bl = r + c * (nrows + 1);
+  switch (inode) {
+  case 0: return bl;
+  case 1: return bl + 1;
+  case 2: return bl + (nrows + 1);
+  default: return bl + (nrows + 2);
+  }
+}
+
+// Local coordinates of a point in cell (c, r).
+struct Local {
+  double xi, eta;
+};
+
+__device__ __forceinline__ Local local_coords(double x, double y, int c,
+                                              int r, double hx, double hy) {
+  Local L;
+  L.xi = (x - c * hx) / hx;
+  L.eta = (y - r * hy) / hy;
+  return L;
+}
+
+// N_inode (gpu_kernels.cpp:33-41)
+__device__ __forceinline__ double shape(const Local &L, int inode) {
+  switch (inode) {
+  case 3: return L.xi * L.eta;
+  case 2: return L.xi * (1.0 - L.eta);
+  case 1: return (1.0 - L.xi) * L.eta;
+  default: return (1.0 - L.xi) * (1.0 - L.eta);
+  }
+}
+
+// dN_inode/dx (gpu_kernels.cpp:43-59, idir 0)
+__device__ __forceinline__ double shape_dx(const Local &L, int inode,
+                                           double hx) {
+  switch (inode) {
+  case 3: return (1.0 / hx) * L.eta;
+  case 2: return (1.0 / hx) * (1.0 - L.eta);
+  case 1: return (-1.0 / hx) * L.eta;
+  default: return (-1.0 / hx) * (1.0 - L.eta);
+  }
+}
+
+// dN_inode/dy (gpu_kernels.cpp:43-59, idir 1)
+__device__ __forceinline__ double shape_dy(const Local &L, int inode,
+                                           double hy) {
+  switch (inode) {
+  case 3: return L.xi * (1.0 / hy);
+  case 2: return L.xi * (-1.0 / hy);
+  case 1: return (1.0 - L.xi) * (1.0 / hy);
+  default: return (1.0 - L.xi) * (-1.0 / hy);
+  }
+}
+
+// Cell of a point: floor(x/hx), floor(y/hy) clamped to the grid.
+__device__ __forceinline__ int cell_of(double x, double y, int nrows,
+                                       int ncols, double hx, double hy) {
+  int c = (int)floor(x / hx), r = (int)floor(y / hy);
+  c = c < 0 ? 0 : (c > ncols - 1 ? ncols - 1 : c);
+  r = r < 0 ? 0 : (r > nrows - 1 ? nrows - 1 : r);
+  return r + c * nrows;
+}
+
+// Wave-wide helpers (64-lane wavefronts).
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+}  // namespace amsc

@@ -1,0 +1,762 @@
+// dropin.hip -- the 18 step kernels of the reference's gpu_kernels.h as
+// gfx950 HIP kernels behind a C ABI (include/amsc_mpm.h, group 1).
+//
+// Per-particle and per-node entry points are one thread per item with the
+// reference's expressions.  The four colour-scattered P2G transfers
+// (gpu_kernels.cpp:97-128,183-246,263-296,572-609) become one node-centric
+// gather each: a node reads the particles of its (up to 4) cells in colour
+// order, each cell's particles in cptcls order, and adds them to its current
+// value.  That is the exact addition sequence the colour loop produces (cells
+// of one colour never share a node), without atomics, without 4 launches and
+// without read-modify-write traffic on the nodal arrays.
+#include "common.hpp"
+#include "amsc_mpm.h"
+
+#include <mutex>
+
+namespace amsc {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Colour of each cell, rebuilt from (ccidx, coloff) on every scatter call.
+struct ColourMap {
+  uint8_t *d = nullptr;
+  size_t cap = 0;
+};
+ColourMap g_colour;
+std::mutex g_colour_mu;
+
+__global__ void fill_u8_kernel(uint8_t *p, int n, uint8_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void colour_map_kernel(const int *ccidx, int nc, int o1, int o2,
+                                  int o3, uint8_t *colour) {
+  const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ic >= nc) return;
+  const int cell = ccidx[ic];
+  if (cell < 0 || cell >= nc) return;
+  colour[cell] = (uint8_t)(ic < o1 ? 0 : ic < o2 ? 1 : ic < o3 ? 2 : 3);
+}
+
+// Validate the colouring and build the colour map; returns ncols in *ncols.
+int prepare_colouring(int nrows, const int *ccidx, const int *coloff,
+                      int *ncols, const uint8_t **colour) {
+  if (nrows <= 0) return fail(AMSC_ERR_INVALID, "nrows must be positive");
+  if (!coloff) return fail(AMSC_ERR_INVALID, "coloff (host int[5]) is NULL");
+  if (coloff[0] != 0)
+    return fail(AMSC_ERR_INVALID, "coloff[0] must be 0");
+  for (int k = 0; k < 4; ++k)
+    if (coloff[k + 1] < coloff[k])
+      return fail(AMSC_ERR_INVALID, "coloff must be non-decreasing");
+  const int nc = coloff[4];
+  if (nc % nrows != 0)
+    return fail(AMSC_ERR_INVALID,
+                "coloff[4] must equal nrows*ncols (ccidx lists every cell)");
+  *ncols = nc / nrows;
+  if (nc == 0) {
+    *colour = nullptr;
+    return AMSC_OK;
+  }
+  std::lock_guard<std::mutex> lk(g_colour_mu);
+  if (g_colour.cap < (size_t)nc) {
+    if (g_colour.d) (void)hipFree(g_colour.d);
+    g_colour.d = nullptr;
+    g_colour.cap = 0;
+    if (hipMalloc(&g_colour.d, (size_t)nc) != hipSuccess)
+      return fail(AMSC_ERR_NOMEM, "colour map allocation failed");
+    g_colour.cap = nc;
+  }
+  hipStream_t st = current_stream();
+  fill_u8_kernel<<<grid_for(nc, kBlock), kBlock, 0, st>>>(g_colour.d, nc, 0xFF);
+  colour_map_kernel<<<grid_for(nc, kBlock), kBlock, 0, st>>>(
+      ccidx, nc, coloff[1], coloff[2], coloff[3], g_colour.d);
+  AMSC_LAUNCH_CHECK();
+  *colour = g_colour.d;
+  return AMSC_OK;
+}
+
+// The (up to 4) cells around node (c, r), sorted by colour; corner is the
+// node's local index in that cell.
+struct NodeCells {
+  int cell[4];
+  int cc[4], cr[4], corner[4];
+  int n;
+};
+
+__device__ __forceinline__ NodeCells node_cells(int c, int r, int nrows,
+                                                int ncols,
+                                                const uint8_t *colour) {
+  NodeCells nc;
+  nc.n = 0;
+  int key[4];
+  // (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r) corner 0
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ccol = c - 1 + (k >> 1);
+    const int crow = r - 1 + (k & 1);
+    if (ccol < 0 || ccol >= ncols || crow < 0 || crow >= nrows) continue;
+    const int cell = crow + ccol * nrows;
+    const int col = colour[cell];
+    if (col == 0xFF) continue;
+    int i = nc.n++;
+    // insertion by (colour, cell)
+    while (i > 0 && (key[i - 1] > col * 4 + k)) {
+      key[i] = key[i - 1];
+      nc.cell[i] = nc.cell[i - 1];
+      nc.cc[i] = nc.cc[i - 1];
+      nc.cr[i] = nc.cr[i - 1];
+      nc.corner[i] = nc.corner[i - 1];
+      --i;
+    }
+    key[i] = col * 4 + k;
+    nc.cell[i] = cell;
+    nc.cc[i] = ccol;
+    nc.cr[i] = crow;
+    nc.corner[i] = 3 - k;
+  }
+  return nc;
+}
+
+template <int NF>
+struct Fields {
+  const double *in[NF];
+  double *out[NF];
+};
+
+// out[f][n] += sum N_corner(x_p) * in[f][p], reference order.
+template <int NF>
+__global__ void __launch_bounds__(kBlock)
+    gather_shape_kernel(int nrows, int ncols, double hx, double hy,
+                        const int *__restrict__ cstart,
+                        const int *__restrict__ cptcls,
+                        const uint8_t *__restrict__ colour,
+                        const double *__restrict__ x,
+                        const double *__restrict__ y, Fields<NF> f) {
+  const int nn = (nrows + 1) * (ncols + 1);
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const int c = n / (nrows + 1), r = n % (nrows + 1);
+  const NodeCells cells = node_cells(c, r, nrows, ncols, colour);
+  if (cells.n == 0) return;
+  double acc[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) acc[k] = f.out[k][n];
+  for (int q = 0; q < cells.n; ++q) {
+    const int cell = cells.cell[q];
+    const int e = cstart[cell + 1];
+    for (int jp = cstart[cell]; jp < e; ++jp) {
+      const int ip = cptcls[jp];
+      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy);
+      const double N = shape(L, cells.corner[q]);
+#pragma unroll
+      for (int k = 0; k < NF; ++k) acc[k] += N * f.in[k][ip];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NF; ++k) f.out[k][n] = acc[k];
+}
+
+// F_int_vx += (Nx*F11 + Ny*F21)*Vp, F_int_vy += (Nx*F12 + Ny*F22)*Vp
+// (gpu_kernels.cpp:286-292)
+__global__ void __launch_bounds__(kBlock)
+    gather_internal_kernel(int nrows, int ncols, double hx, double hy,
+                           const int *__restrict__ cstart,
+                           const int *__restrict__ cptcls,
+                           const uint8_t *__restrict__ colour,
+                           const double *__restrict__ x,
+                           const double *__restrict__ y,
+                           const double *__restrict__ Vp,
+                           const double *__restrict__ F11,
+                           const double *__restrict__ F12,
+                           const double *__restrict__ F21,
+                           const double *__restrict__ F22, double *Fix,
+                           double *Fiy) {
+  const int nn = (nrows + 1) * (ncols + 1);
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const int c = n / (nrows + 1), r = n % (nrows + 1);
+  const NodeCells cells = node_cells(c, r, nrows, ncols, colour);
+  if (cells.n == 0) return;
+  double ax = Fix[n], ay = Fiy[n];
+  for (int q = 0; q < cells.n; ++q) {
+    const int cell = cells.cell[q];
+    const int e = cstart[cell + 1];
+    for (int jp = cstart[cell]; jp < e; ++jp) {
+      const int ip = cptcls[jp];
+      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy);
+      const double Nx = shape_dx(L, cells.corner[q], hx);
+      const double Ny = shape_dy(L, cells.corner[q], hy);
+      const double vp = Vp[ip];
+      ax += (Nx * F11[ip] + Ny * F21[ip]) * vp;
+      ay += (Nx * F12[ip] + Ny * F22[ip]) * vp;
+    }
+  }
+  Fix[n] = ax;
+  Fiy[n] = ay;
+}
+
+template <int NF>
+int gather_shape(int nrows, double hx, double hy, const int *cstart,
+                 const int *cptcls, const int *ccidx, const int *coloff,
+                 const double *x, const double *y, Fields<NF> f) {
+  int ncols = 0;
+  const uint8_t *colour = nullptr;
+  int rc = prepare_colouring(nrows, ccidx, coloff, &ncols, &colour);
+  if (rc) return rc;
+  if (ncols == 0) return AMSC_OK;
+  const int nn = (nrows + 1) * (ncols + 1);
+  gather_shape_kernel<NF><<<grid_for(nn, kBlock), kBlock, 0, current_stream()>>>(
+      nrows, ncols, hx, hy, cstart, cptcls, colour, x, y, f);
+  AMSC_LAUNCH_CHECK();
+  return AMSC_OK;
+}
+
+struct Ptrs25 {
+  double *p[25];
+};
+
+__global__ void reset_kernel(int nn, Ptrs25 a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+#pragma unroll
+  for (int k = 0; k < 25; ++k) a.p[k][i] = 0;
+}
+
+// gpu_kernels.cpp:131-152
+__global__ void gradZ_kernel(int np, int nrows, double hx, double hy,
+                             const double *__restrict__ x,
+                             const double *__restrict__ y,
+                             const int *__restrict__ p2g,
+                             const double *__restrict__ Z, double *dZxp,
+                             double *dZyp) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  double dzx = 0.0, dzy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double z = Z[node_id(k, c, r, nrows)];
+    dzx += shape_dx(L, k, hx) * z;
+    dzy += shape_dy(L, k, hy) * z;
+  }
+  dZxp[ip] = dzx;
+  dZyp[ip] = dzy;
+}
+
+// gpu_kernels.cpp:155-167
+__global__ void gravity_kernel(int np, double g, const double *__restrict__ hp,
+                               const double *__restrict__ H,
+                               const double *__restrict__ Mp,
+                               const double *__restrict__ dZxp,
+                               const double *__restrict__ dZyp, double *Fpx,
+                               double *Fpy) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const double ratio = hp[ip] > 1e-10 ? H[ip] / hp[ip] : 0.0;
+  Fpx[ip] = -g * Mp[ip] * (1.0 - ratio) * dZxp[ip];
+  Fpy[ip] = -g * Mp[ip] * (1.0 - ratio) * dZyp[ip];
+}
+
+// gpu_kernels.cpp:170-180
+__global__ void friction_kernel(int np, const double *__restrict__ Ap,
+                                const double *__restrict__ Fbx,
+                                const double *__restrict__ Fby, double *Fx,
+                                double *Fy) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  Fx[ip] = Ap[ip] * Fbx[ip];
+  Fy[ip] = Ap[ip] * Fby[ip];
+}
+
+// gpu_kernels.cpp:249-260
+__global__ void fext_kernel(int nn, const double *__restrict__ FPx,
+                            const double *__restrict__ FPy,
+                            const double *__restrict__ Frx,
+                            const double *__restrict__ Fry, double *Fx,
+                            double *Fy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  Fx[i] = FPx[i] + Frx[i];
+  Fy[i] = FPy[i] + Fry[i];
+}
+
+// gpu_kernels.cpp:299-315 (momentum update commented out in the reference)
+__global__ void ftot_kernel(int nn, const double *__restrict__ Fex,
+                            const double *__restrict__ Fey,
+                            const double *__restrict__ Fix,
+                            const double *__restrict__ Fiy, double *Ftx,
+                            double *Fty) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  Ftx[i] = Fex[i] + Fix[i];
+  Fty[i] = Fey[i] + Fiy[i];
+}
+
+// gpu_kernels.cpp:318-353
+__global__ void nodal_vel_bc_kernel(int nn, int nrows, int ncols, double dt,
+                                    int bc, const double *__restrict__ Mv,
+                                    const double *__restrict__ Ftx,
+                                    const double *__restrict__ Fty,
+                                    double *momx, double *momy, double *avx,
+                                    double *avy, double *vvx, double *vvy,
+                                    double *vvxL, double *vvyL) {
+  const int iv = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iv >= nn) return;
+  const double m = Mv[iv];
+  const bool active = m > 1e-2;
+  double ax = active ? Ftx[iv] / m : 0.0;
+  double ay = active ? Fty[iv] / m : 0.0;
+  double vx = active ? momx[iv] / m : 0.0;
+  double vy = active ? momy[iv] / m : 0.0;
+  double vxL = active ? dt * ax + vx : 0.0;
+  double vyL = active ? dt * ay + vy : 0.0;
+  if (bc) {
+    const int r = iv % (nrows + 1), c = iv / (nrows + 1);
+    if (c == 0 || c == 1 || c == ncols - 1 || c == ncols) {
+      momx[iv] = 0.0; vx = 0.0; vxL = 0.0; ax = 0.0;
+    }
+    if (r == 0 || r == 1 || r == nrows - 1 || r == nrows) {
+      momy[iv] = 0.0; vy = 0.0; vyL = 0.0; ay = 0.0;
+    }
+  }
+  avx[iv] = ax; avy[iv] = ay;
+  vvx[iv] = vx; vvy[iv] = vy;
+  vvxL[iv] = vxL; vvyL[iv] = vyL;
+}
+
+// gpu_kernels.cpp:356-407
+__global__ void g2p_advect_kernel(int np, int nrows, int ncols, double hx,
+                                  double hy, double dt,
+                                  const int *__restrict__ p2g,
+                                  const double *__restrict__ vvxL,
+                                  const double *__restrict__ vvyL,
+                                  const double *__restrict__ avx,
+                                  const double *__restrict__ avy, double *x,
+                                  double *y, double *vpx, double *vpy,
+                                  double *apx, double *apy, double *mpx,
+                                  double *mpy, double *vpxL, double *vpyL,
+                                  const double *__restrict__ Mp) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  double ax = 0, ay = 0, vxL = 0, vyL = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double N = shape(L, k);
+    const int n = node_id(k, c, r, nrows);
+    ax += N * avx[n];
+    ay += N * avy[n];
+    vxL += N * vvxL[n];
+    vyL += N * vvyL[n];
+  }
+  vpx[ip] = vxL; vpy[ip] = vyL;
+  apx[ip] = ax; apy[ip] = ay;
+  double xn = x[ip] + dt * vxL, yn = y[ip] + dt * vyL;
+  const double eps = 1e-10, Lx = ncols * hx, Ly = nrows * hy;
+  if (xn < eps) xn = eps;
+  if (xn > Lx - eps) xn = Lx - eps;
+  if (yn < eps) yn = eps;
+  if (yn > Ly - eps) yn = Ly - eps;
+  x[ip] = xn; y[ip] = yn;
+  vpxL[ip] = vxL; vpyL[ip] = vyL;
+  mpx[ip] = vxL * Mp[ip];
+  mpy[ip] = vyL * Mp[ip];
+}
+
+// gpu_kernels.cpp:410-454
+__global__ void g2pd_height_kernel(int np, int nrows, double hx, double hy,
+                                   double dt, const double *__restrict__ x,
+                                   const double *__restrict__ y,
+                                   const int *__restrict__ p2g,
+                                   const double *__restrict__ vvxL,
+                                   const double *__restrict__ vvyL,
+                                   double *vxdx_o, double *vxdy_o,
+                                   double *vydx_o, double *vydy_o, double *hp,
+                                   const double *Vp, double *Ap) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  double vxdx = 0, vxdy = 0, vydx = 0, vydy = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int n = node_id(k, c, r, nrows);
+    const double Nx = shape_dx(L, k, hx), Ny = shape_dy(L, k, hy);
+    vxdx += Nx * vvxL[n];
+    vxdy += Ny * vvxL[n];
+    vydx += Nx * vvyL[n];
+    vydy += Ny * vvyL[n];
+  }
+  vxdx_o[ip] = vxdx; vxdy_o[ip] = vxdy;
+  vydx_o[ip] = vydx; vydy_o[ip] = vydy;
+  const double sc = 1.0 + dt * (vxdx + vydy);
+  const double h = hp[ip] / sc;
+  hp[ip] = h;
+  Ap[ip] = Vp[ip] / h;
+}
+
+// gpu_kernels.cpp:457-475
+__global__ void voellmy_kernel(int np, double rho, double g, double tan_fa,
+                               double xi, double fric_on,
+                               const double *__restrict__ vpx,
+                               const double *__restrict__ vpy,
+                               const double *__restrict__ hp, double *Fbx,
+                               double *Fby) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const double vx = vpx[ip], vy = vpy[ip], h = hp[ip];
+  const double nv = sqrt(vx * vx + vy * vy);
+  if (fric_on > 0 && nv > 1e-10 && xi > 0) {
+    const double v2 = vx * vx + vy * vy;
+    Fbx[ip] = -fric_on * (rho * g * h * tan_fa + rho * g * v2 / xi) * vx / nv;
+    Fby[ip] = -fric_on * (rho * g * h * tan_fa + rho * g * v2 / xi) * vy / nv;
+  } else {
+    Fbx[ip] = 0.0;
+    Fby[ip] = 0.0;
+  }
+}
+
+// gpu_kernels.cpp:478-546
+__global__ void bingham_gpu_kernel(
+    int np, double rho, double g, double mu, double tau_Y, double cc,
+    double A_coeff, double C_coeff, const double *__restrict__ vpx,
+    const double *__restrict__ vpy, const double *__restrict__ hp,
+    const double *__restrict__ H, const double *__restrict__ vxdx,
+    const double *__restrict__ vxdy, const double *__restrict__ vydx,
+    const double *__restrict__ vydy, double *F11, double *F12, double *F21,
+    double *F22) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const double vx = vpx[ip], vy = vpy[ip], h = hp[ip];
+  const double nv = sqrt(vx * vx + vy * vy);
+  const double alf = (h > 1e-3 && tau_Y > 1e-12)
+                         ? (6.0 * mu * nv) / ((h + 0.001) * tau_Y)
+                         : 0.0;
+  const double bv = -114.0 / 32.0 - alf;
+  const double sq = sqrt(bv * bv - 4.0 * A_coeff * C_coeff);
+  const double z1 = (-bv + sq) / (2.0 * A_coeff);
+  const double z2 = (-bv - sq) / (2.0 * A_coeff);
+  const double zz = (fabs(z1 - 0.5) <= 0.5) ? z1 : z2;
+  const double Dxx = vxdx[ip];
+  const double Dxy = 0.5 * (vxdy[ip] + vydx[ip]);
+  const double Dyy = vydy[ip];
+  const double Dzz = -(Dxx + Dyy);
+  const double Dzx = h > 1e-3 ? 0.5 * (3.0 / (2.0 + zz)) * (vx / (h + 0.001)) : 0.0;
+  const double Dzy = h > 1e-3 ? 0.5 * (3.0 / (2.0 + zz)) * (vy / (h + 0.001)) : 0.0;
+  const double inv2 = 0.5 * (Dxx * Dxx + Dyy * Dyy + Dzz * Dzz) + Dzx * Dzx +
+                      Dzy * Dzy + Dxy * Dxy;
+  const double coeff_max = 5.0e6;
+  double inv2_floor = (tau_Y / coeff_max) * (tau_Y / coeff_max);
+  if (inv2_floor < 1e-9) inv2_floor = 1e-9;
+  const double coeff = tau_Y / sqrt(inv2 + inv2_floor) + 2.0 * mu;
+  const double h_corr = h > 1e-10 ? h - H[ip] * H[ip] / h : 0.0;
+  F11[ip] = -cc * coeff * Dxx + 0.5 * rho * g * h_corr;
+  F22[ip] = -cc * coeff * Dyy + 0.5 * rho * g * h_corr;
+  F12[ip] = -cc * coeff * Dxy;
+  F21[ip] = -cc * coeff * Dxy;
+}
+
+// gpu_kernels.cpp:549-569
+__global__ void g2p_height_kernel(int np, int nrows, double hx, double hy,
+                                  const double *__restrict__ x,
+                                  const double *__restrict__ y,
+                                  const int *__restrict__ p2g,
+                                  const double *__restrict__ Z,
+                                  const double *__restrict__ hp, double *Zp,
+                                  double *hpZ) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  double zp = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) zp += shape(L, k) * Z[node_id(k, c, r, nrows)];
+  Zp[ip] = zp;
+  hpZ[ip] = hp[ip] + zp;
+}
+
+// gpu_kernels.cpp:612-627
+__global__ void normalize_kernel(int nn, const double *__restrict__ rho,
+                                 double *vx, double *vy, double *ax,
+                                 double *ay) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const double mv = rho[i];
+  if (mv > 1e-8) {
+    vx[i] /= mv;
+    vy[i] /= mv;
+    ax[i] /= mv;
+    ay[i] /= mv;
+  }
+}
+
+int check_n(int n, const char *what) {
+  if (n < 0) return fail(AMSC_ERR_INVALID, std::string(what) + " is negative");
+  return AMSC_OK;
+}
+
+}  // namespace
+}  // namespace amsc
+
+using namespace amsc;
+
+#define LAUNCH1D(kernel, n, ...)                                             \
+  do {                                                                       \
+    if ((n) > 0) {                                                           \
+      kernel<<<grid_for((n), kBlock), kBlock, 0, current_stream()>>>(        \
+          __VA_ARGS__);                                                      \
+      AMSC_LAUNCH_CHECK();                                                   \
+    }                                                                        \
+  } while (0)
+
+extern "C" {
+
+int amsc_reset_nodal_arrays(int nn, double *d_Mv, double *d_mom_vx,
+                            double *d_mom_vy, double *d_F_ext_vx,
+                            double *d_F_ext_vy, double *d_F_int_vx,
+                            double *d_F_int_vy, double *d_Ftot_vx,
+                            double *d_Ftot_vy, double *d_avx, double *d_avy,
+                            double *d_vvx, double *d_vvy, double *d_vvxL,
+                            double *d_vvyL, double *d_Fric_x, double *d_Fric_y,
+                            double *d_FPxv, double *d_FPyv, double *d_HV,
+                            double *d_pv_rho, double *d_pv_vvx,
+                            double *d_pv_vvy, double *d_pv_avx,
+                            double *d_pv_avy) {
+  if (int rc = check_n(nn, "nn")) return rc;
+  Ptrs25 a = {{d_Mv, d_mom_vx, d_mom_vy, d_F_ext_vx, d_F_ext_vy, d_F_int_vx,
+               d_F_int_vy, d_Ftot_vx, d_Ftot_vy, d_avx, d_avy, d_vvx, d_vvy,
+               d_vvxL, d_vvyL, d_Fric_x, d_Fric_y, d_FPxv, d_FPyv, d_HV,
+               d_pv_rho, d_pv_vvx, d_pv_vvy, d_pv_avx, d_pv_avy}};
+  LAUNCH1D(reset_kernel, nn, nn, a);
+  return AMSC_OK;
+}
+
+int amsc_scatter_mass_momentum(int np, int nrows, double hx, double hy,
+                               const int *cstart, const int *cptcls,
+                               const int *ccidx, const int *coloff,
+                               const double *d_x, const double *d_y,
+                               const double *d_Mp, const double *d_mom_px,
+                               const double *d_mom_py, double *d_Mv,
+                               double *d_mom_vx, double *d_mom_vy) {
+  (void)np;
+  Fields<3> f = {{d_Mp, d_mom_px, d_mom_py}, {d_Mv, d_mom_vx, d_mom_vy}};
+  return gather_shape<3>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+                         d_y, f);
+}
+
+int amsc_gradZ_to_particles(int np, int nrows, double hx, double hy,
+                            const double *d_x, const double *d_y,
+                            const int *d_p2g, const double *d_Z,
+                            double *d_dZxp, double *d_dZyp) {
+  if (int rc = check_n(np, "np")) return rc;
+  LAUNCH1D(gradZ_kernel, np, np, nrows, hx, hy, d_x, d_y, d_p2g, d_Z, d_dZxp,
+           d_dZyp);
+  return AMSC_OK;
+}
+
+int amsc_gravity_force(int np, double g_c, const double *d_hp,
+                       const double *d_H, const double *d_Mp,
+                       const double *d_dZxp, const double *d_dZyp,
+                       double *d_Fpx, double *d_Fpy) {
+  if (int rc = check_n(np, "np")) return rc;
+  LAUNCH1D(gravity_kernel, np, np, g_c, d_hp, d_H, d_Mp, d_dZxp, d_dZyp,
+           d_Fpx, d_Fpy);
+  return AMSC_OK;
+}
+
+int amsc_friction_particle_force(int np, const double *d_Ap,
+                                 const double *d_Fb_x, const double *d_Fb_y,
+                                 double *d_Fric_px, double *d_Fric_py) {
+  if (int rc = check_n(np, "np")) return rc;
+  LAUNCH1D(friction_kernel, np, np, d_Ap, d_Fb_x, d_Fb_y, d_Fric_px,
+           d_Fric_py);
+  return AMSC_OK;
+}
+
+int amsc_scatter_gravity_force(int np, int nrows, double hx, double hy,
+                               const int *cstart, const int *cptcls,
+                               const int *ccidx, const int *coloff,
+                               const double *d_x, const double *d_y,
+                               const double *d_Fpx, const double *d_Fpy,
+                               double *d_FPxv, double *d_FPyv) {
+  (void)np;
+  Fields<2> f = {{d_Fpx, d_Fpy}, {d_FPxv, d_FPyv}};
+  return gather_shape<2>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+                         d_y, f);
+}
+
+int amsc_scatter_friction_force(int np, int nrows, double hx, double hy,
+                                const int *cstart, const int *cptcls,
+                                const int *ccidx, const int *coloff,
+                                const double *d_x, const double *d_y,
+                                const double *d_Fric_px,
+                                const double *d_Fric_py, double *d_Fric_x,
+                                double *d_Fric_y) {
+  (void)np;
+  Fields<2> f = {{d_Fric_px, d_Fric_py}, {d_Fric_x, d_Fric_y}};
+  return gather_shape<2>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+                         d_y, f);
+}
+
+int amsc_compute_F_ext(int nn, const double *d_FPxv, const double *d_FPyv,
+                       const double *d_Fric_x, const double *d_Fric_y,
+                       double *d_F_ext_vx, double *d_F_ext_vy) {
+  if (int rc = check_n(nn, "nn")) return rc;
+  LAUNCH1D(fext_kernel, nn, nn, d_FPxv, d_FPyv, d_Fric_x, d_Fric_y,
+           d_F_ext_vx, d_F_ext_vy);
+  return AMSC_OK;
+}
+
+int amsc_scatter_internal_forces(int np, int nrows, double hx, double hy,
+                                 const int *cstart, const int *cptcls,
+                                 const int *ccidx, const int *coloff,
+                                 const double *d_x, const double *d_y,
+                                 const double *d_Vp, const double *d_F11,
+                                 const double *d_F12, const double *d_F21,
+                                 const double *d_F22, double *d_F_int_vx,
+                                 double *d_F_int_vy) {
+  (void)np;
+  int ncols = 0;
+  const uint8_t *colour = nullptr;
+  if (int rc = prepare_colouring(nrows, ccidx, coloff, &ncols, &colour))
+    return rc;
+  if (ncols == 0) return AMSC_OK;
+  const int nn = (nrows + 1) * (ncols + 1);
+  LAUNCH1D(gather_internal_kernel, nn, nrows, ncols, hx, hy, cstart, cptcls,
+           colour, d_x, d_y, d_Vp, d_F11, d_F12, d_F21, d_F22, d_F_int_vx,
+           d_F_int_vy);
+  return AMSC_OK;
+}
+
+int amsc_nodal_force_and_momentum(int nn, double dt, const double *d_F_ext_vx,
+                                  const double *d_F_ext_vy,
+                                  const double *d_F_int_vx,
+                                  const double *d_F_int_vy, double *d_Ftot_vx,
+                                  double *d_Ftot_vy, double *d_mom_vx,
+                                  double *d_mom_vy) {
+  (void)dt; (void)d_mom_vx; (void)d_mom_vy;
+  if (int rc = check_n(nn, "nn")) return rc;
+  LAUNCH1D(ftot_kernel, nn, nn, d_F_ext_vx, d_F_ext_vy, d_F_int_vx,
+           d_F_int_vy, d_Ftot_vx, d_Ftot_vy);
+  return AMSC_OK;
+}
+
+int amsc_nodal_velocity_and_bc(int nn, int nrows, int ncols, double dt,
+                               int bc_flag, const double *d_Mv,
+                               const double *d_Ftot_vx,
+                               const double *d_Ftot_vy, double *d_mom_vx,
+                               double *d_mom_vy, double *d_avx, double *d_avy,
+                               double *d_vvx, double *d_vvy, double *d_vvxL,
+                               double *d_vvyL) {
+  if (int rc = check_n(nn, "nn")) return rc;
+  if (nrows <= 0) return fail(AMSC_ERR_INVALID, "nrows must be positive");
+  LAUNCH1D(nodal_vel_bc_kernel, nn, nn, nrows, ncols, dt, bc_flag ? 1 : 0,
+           d_Mv, d_Ftot_vx, d_Ftot_vy, d_mom_vx, d_mom_vy, d_avx, d_avy,
+           d_vvx, d_vvy, d_vvxL, d_vvyL);
+  return AMSC_OK;
+}
+
+int amsc_g2p_velocity_and_advect(int np, int nrows, int ncols, double hx,
+                                 double hy, double dt, const int *d_p2g,
+                                 const double *d_vvxL, const double *d_vvyL,
+                                 const double *d_avx, const double *d_avy,
+                                 double *d_x, double *d_y, double *d_vpx,
+                                 double *d_vpy, double *d_apx, double *d_apy,
+                                 double *d_mom_px, double *d_mom_py,
+                                 double *d_vpxL, double *d_vpyL,
+                                 const double *d_Mp) {
+  if (int rc = check_n(np, "np")) return rc;
+  if (nrows <= 0) return fail(AMSC_ERR_INVALID, "nrows must be positive");
+  LAUNCH1D(g2p_advect_kernel, np, np, nrows, ncols, hx, hy, dt, d_p2g, d_vvxL,
+           d_vvyL, d_avx, d_avy, d_x, d_y, d_vpx, d_vpy, d_apx, d_apy,
+           d_mom_px, d_mom_py, d_vpxL, d_vpyL, d_Mp);
+  return AMSC_OK;
+}
+
+int amsc_g2pd_gradients_and_height_update(
+    int np, int nrows, double hx, double hy, double dt, const double *d_x,
+    const double *d_y, const int *d_p2g, const double *d_vvxL,
+    const double *d_vvyL, double *d_vpx_dx, double *d_vpx_dy,
+    double *d_vpy_dx, double *d_vpy_dy, double *d_hp, double *d_Vp,
+    double *d_Ap, double *min_sc_out, double *min_hp_out) {
+  (void)min_sc_out; (void)min_hp_out;
+  if (int rc = check_n(np, "np")) return rc;
+  if (nrows <= 0) return fail(AMSC_ERR_INVALID, "nrows must be positive");
+  LAUNCH1D(g2pd_height_kernel, np, np, nrows, hx, hy, dt, d_x, d_y, d_p2g,
+           d_vvxL, d_vvyL, d_vpx_dx, d_vpx_dy, d_vpy_dx, d_vpy_dy, d_hp, d_Vp,
+           d_Ap);
+  return AMSC_OK;
+}
+
+int amsc_voellmy_friction(int np, double rho, double g_c, double tan_fa,
+                          double xi, double fric_on, const double *d_vpx,
+                          const double *d_vpy, const double *d_hp,
+                          double *d_Fb_x, double *d_Fb_y) {
+  if (int rc = check_n(np, "np")) return rc;
+  LAUNCH1D(voellmy_kernel, np, np, rho, g_c, tan_fa, xi, fric_on, d_vpx,
+           d_vpy, d_hp, d_Fb_x, d_Fb_y);
+  return AMSC_OK;
+}
+
+int amsc_bingham_stress_update(
+    int np, double rho, double g_c, double mu, double tau_Y, double cc,
+    double A_coeff, double C_coeff, const double *d_vpx, const double *d_vpy,
+    const double *d_hp, const double *d_H, const double *d_vpx_dx,
+    const double *d_vpx_dy, const double *d_vpy_dx, const double *d_vpy_dy,
+    double *d_F11, double *d_F12, double *d_F21, double *d_F22,
+    double *max_coeff_out, double *min_hp_out, double *max_nv_out,
+    double *max_F11_out, double *max_dev_out, double *max_pres_out,
+    double *max_Dxx_out) {
+  (void)max_coeff_out; (void)min_hp_out; (void)max_nv_out; (void)max_F11_out;
+  (void)max_dev_out; (void)max_pres_out; (void)max_Dxx_out;
+  if (int rc = check_n(np, "np")) return rc;
+  LAUNCH1D(bingham_gpu_kernel, np, np, rho, g_c, mu, tau_Y, cc, A_coeff,
+           C_coeff, d_vpx, d_vpy, d_hp, d_H, d_vpx_dx, d_vpx_dy, d_vpy_dx,
+           d_vpy_dy, d_F11, d_F12, d_F21, d_F22);
+  return AMSC_OK;
+}
+
+int amsc_g2p_height(int np, int nrows, double hx, double hy,
+                    const double *d_x, const double *d_y, const int *d_p2g,
+                    const double *d_Z, const double *d_hp, double *d_Zp,
+                    double *d_hpZ) {
+  if (int rc = check_n(np, "np")) return rc;
+  if (nrows <= 0) return fail(AMSC_ERR_INVALID, "nrows must be positive");
+  LAUNCH1D(g2p_height_kernel, np, np, nrows, hx, hy, d_x, d_y, d_p2g, d_Z,
+           d_hp, d_Zp, d_hpZ);
+  return AMSC_OK;
+}
+
+int amsc_scatter_plotvars(int np, int nrows, double hx, double hy,
+                          const int *cstart, const int *cptcls,
+                          const int *ccidx, const int *coloff,
+                          const double *d_x, const double *d_y,
+                          const double *d_Mp, const double *d_vpx,
+                          const double *d_vpy, const double *d_apx,
+                          const double *d_apy, const double *d_hp,
+                          double *d_pv_rho, double *d_pv_vvx,
+                          double *d_pv_vvy, double *d_pv_avx,
+                          double *d_pv_avy, double *d_HV) {
+  (void)np;
+  Fields<6> f = {{d_Mp, d_vpx, d_vpy, d_apx, d_apy, d_hp},
+                 {d_pv_rho, d_pv_vvx, d_pv_vvy, d_pv_avx, d_pv_avy, d_HV}};
+  return gather_shape<6>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+                         d_y, f);
+}
+
+int amsc_normalize_plotvars(int nn, double *d_pv_rho, double *d_pv_vvx,
+                            double *d_pv_vvy, double *d_pv_avx,
+                            double *d_pv_avy) {
+  if (int rc = check_n(nn, "nn")) return rc;
+  LAUNCH1D(normalize_kernel, nn, nn, d_pv_rho, d_pv_vvx, d_pv_vvy, d_pv_avx,
+           d_pv_avy);
+  return AMSC_OK;
+}
+
+}  // extern "C"

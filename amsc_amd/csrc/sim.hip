@@ -1,0 +1,1312 @@
+// sim.hip -- device-resident depth-averaged MPM time loop (FP64, optim_par
+// semantics) on gfx950.
+//
+// One step (main/optim_par.cpp:281-668, merge/split at the position of
+// main/gpu_offload.cpp:321-401):
+//   prologue         1 thread: end test, dt (fixed or CFL from the device
+//                    maxima of the previous step), t, it.
+//   [merge/split]    flag / pair (merge_split.hip), weight scan, execute
+//                    into the other SoA buffer, re-bin.
+//   p2g_nodal        tiles of 16x16 nodes: the particles of the tile's
+//                    17x17 cells (CSR ranges) are staged through LDS in
+//                    chunks of 256; each thread then folds its node's
+//                    contributions in ascending cell, ascending id order
+//                    (the serial p2g order, hence bit-identical sums),
+//                    and applies the nodal update + boundary conditions in
+//                    registers.  The 9 P2G accumulators never touch HBM;
+//                    only vL (and a, on output steps) are written.
+//   particle         1 thread per particle: G2P, PIC velocity, advection,
+//                    G2PD at the new position with the step-start cell,
+//                    height update, Voellmy, Bingham, friction force, the
+//                    new cell key + wave-aggregated count, CFL maxima.
+//   binning          scan of the counts -> cstart, placement, in-cell sort.
+// FMA contraction is off for this file: every expression is evaluated in
+// the oracle's order, so results match the FP64 CPU path bit for bit.
+#include "binning.hpp"
+#include "common.hpp"
+#include "merge_split.hpp"
+#include "amsc_mpm.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace amsc {
+
+// ------------------------------------------------------------- layout
+enum PF : int {
+  // persistent state
+  PF_X, PF_Y, PF_VPX, PF_VPY, PF_HP, PF_VP, PF_MP, PF_H, PF_F11, PF_F12,
+  PF_F22, PF_FRX, PF_FRY,
+  // written on output / merge steps only
+  PF_AP, PF_FBX, PF_FBY, PF_VXDX, PF_VXDY, PF_VYDX, PF_VYDY, PF_APX, PF_APY,
+  PF_ZP, PF_HPZ, PF_DZX, PF_DZY, PF_FPX, PF_FPY, PF_MOMX, PF_MOMY,
+  PF_COUNT
+};
+constexpr int PF_CORE = PF_AP;
+
+static const char *kPFName[PF_COUNT] = {
+    "x",      "y",      "vpx",    "vpy",     "hp",      "Vp",     "Mp",
+    "H",      "F_11",   "F_12",   "F_22",    "Fric_px", "Fric_py", "Ap",
+    "Fb_x",   "Fb_y",   "vpx_dx", "vpx_dy",  "vpy_dx",  "vpy_dy", "apx",
+    "apy",    "Zp",     "hpZ",    "dZxp",    "dZyp",    "Fpx",    "Fpy",
+    "mom_px", "mom_py"};
+
+__host__ __device__ constexpr bool pf_extensive(int k) {
+  // merge_split_ops_cmes.h:49-50
+  return k == PF_MP || k == PF_VP || k == PF_AP || k == PF_MOMX ||
+         k == PF_MOMY;
+}
+
+enum NF : int {
+  NF_VVXL, NF_VVYL,  // always
+  NF_AVX, NF_AVY, NF_MV, NF_MOMX, NF_MOMY, NF_FEXTX, NF_FEXTY, NF_FINTX,
+  NF_FINTY, NF_FRX, NF_FRY, NF_FPX, NF_FPY, NF_FTOTX, NF_FTOTY, NF_VVX,
+  NF_VVY,
+  NF_COUNT
+};
+static const char *kNFName[NF_COUNT] = {
+    "vvxL",   "vvyL",     "avx",      "avy",      "Mv",      "mom_vx",
+    "mom_vy", "F_ext_vx", "F_ext_vy", "F_int_vx", "F_int_vy", "Fric_x",
+    "Fric_y", "FPxv",     "FPyv",     "Ftot_vx",  "Ftot_vy",  "vvx",
+    "vvy"};
+
+struct PPtr {
+  double *f[PF_COUNT];
+  int *label, *level;
+};
+struct NPtr {
+  double *f[NF_COUNT];
+};
+
+// Device control block.
+struct Ctrl {
+  double t;        // time after the last step taken
+  double dt;       // dt of the last step taken
+  double t_start;  // time at the start of the last step taken
+  double cons[2];  // err_mass, err_mom of the last step
+  unsigned long long cfl[2][3];  // max |vpx|, |vpy|, hp bits (by step parity)
+  int it;
+  int active;
+};
+
+struct KParams {
+  int nrows, ncols;
+  double hx, hy, g, rho, mu, tauy, xi, tan_fa, bing, fric, cfl, eq_level,
+      dt_fixed, T;
+  int bc, nsteps, max_steps;
+};
+
+// ------------------------------------------------------------- kernels
+namespace {
+
+constexpr int kTile = 16;              // nodes per tile side
+constexpr int kSeg = kTile + 1;        // cell columns (incl. halo)
+constexpr int kChunk = 256;            // particles staged per round
+constexpr int kP2GThreads = kTile * kTile;
+constexpr int kNS = 13;                // staged values per particle
+
+__global__ void prologue_kernel(KParams P, Ctrl *ctrl) {
+  if (!ctrl->active) return;
+  const bool fixed = P.dt_fixed > 0.0;
+  const int it = ctrl->it;
+  // loop condition, optim_par.cpp:281; optional step cap for adaptive runs
+  if (fixed ? !(it < P.nsteps) : !(ctrl->t < P.T)) {
+    ctrl->active = 0;
+    return;
+  }
+  if (!fixed && P.max_steps > 0 && it >= P.max_steps) {
+    ctrl->active = 0;
+    return;
+  }
+  double dt = ctrl->dt;
+  const int s = it + 1;  // this step
+  if (fixed) {
+    dt = P.dt_fixed;
+  } else if (it > 0) {
+    // optim_par.cpp:289-304: max |v| and max hp over the particles
+    const double mvx = __longlong_as_double((long long)ctrl->cfl[(s - 1) & 1][0]);
+    const double mvy = __longlong_as_double((long long)ctrl->cfl[(s - 1) & 1][1]);
+    const double hmax = __longlong_as_double((long long)ctrl->cfl[(s - 1) & 1][2]);
+    const double a = sqrt(P.g * hmax) + mvx, b = sqrt(P.g * hmax) + mvy;
+    const double cel = fabs(a > b ? a : b);
+    const double minh = P.hx < P.hy ? P.hx : P.hy;
+    dt = P.cfl * minh / (1e-2 + cel);
+  }
+  ctrl->dt = dt;
+  ctrl->t_start = ctrl->t;
+  ctrl->t = ctrl->t + dt;
+  ctrl->it = s;
+  ctrl->cfl[s & 1][0] = ctrl->cfl[s & 1][1] = ctrl->cfl[s & 1][2] = 0ull;
+}
+
+// Step-0 initialisation, optim_par.cpp:235-264 (cells from the initial
+// binning): Zp, hpZ, H, hydrostatic F, Fric = Ap*Fb with Fb = 0.
+__global__ void init_kernel(int np, KParams P, PPtr p, const int *key,
+                            const double *__restrict__ Z,
+                            const double *mom_x_in, const double *mom_y_in) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  const int cell = key[ip], r = cell % P.nrows, c = cell / P.nrows;
+  const Local L = local_coords(p.f[PF_X][ip], p.f[PF_Y][ip], c, r, P.hx, P.hy);
+  double zp = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) zp += shape(L, k) * Z[node_id(k, c, r, P.nrows)];
+  const double hp = p.f[PF_HP][ip];
+  p.f[PF_ZP][ip] = zp;
+  p.f[PF_HPZ][ip] = hp + zp;
+  const double diff = P.eq_level - zp;
+  const double H = (P.eq_level > 0 && diff > 0) ? diff : 0.0;
+  p.f[PF_H][ip] = H;
+  const double h_corr = hp > 1e-10 ? hp - (H * H) / hp : 0.0;
+  p.f[PF_F11][ip] = -.5 * P.rho * P.g * h_corr;
+  p.f[PF_F12][ip] = 0.0;
+  p.f[PF_F22][ip] = -.5 * P.rho * P.g * h_corr;
+  p.f[PF_FBX][ip] = 0.0;
+  p.f[PF_FBY][ip] = 0.0;
+  p.f[PF_FRX][ip] = p.f[PF_AP][ip] * 0.0;
+  p.f[PF_FRY][ip] = p.f[PF_AP][ip] * 0.0;
+  const int zero_fields[] = {PF_VXDX, PF_VXDY, PF_VYDX, PF_VYDY, PF_APX,
+                             PF_APY,  PF_DZX,  PF_DZY,  PF_FPX,  PF_FPY};
+#pragma unroll
+  for (int k = 0; k < 10; ++k) p.f[zero_fields[k]][ip] = 0.0;
+  p.f[PF_MOMX][ip] = mom_x_in[ip];
+  p.f[PF_MOMY][ip] = mom_y_in[ip];
+  p.label[ip] = ip;
+  p.level[ip] = 0;
+}
+
+struct P2GArgs {
+  KParams P;
+  const int *cstart, *cptcls;
+  PPtr p;
+  const double *Z;
+  const Ctrl *ctrl;
+  NPtr n;
+  double *partials;  // [nblocks][6] or null
+  int aux;           // write avx/avy, nodal dump and P2G particle outputs
+  int mom_stored;    // momentum from the mom arrays (first step only)
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+__global__ void __launch_bounds__(kP2GThreads)
+    p2g_nodal_kernel(P2GArgs a) {
+  const Ctrl *ctrl = a.ctrl;
+  if (!ctrl->active) return;
+  const KParams &P = a.P;
+  const int nrows = P.nrows, ncols = P.ncols;
+  const double hx = P.hx, hy = P.hy;
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.x * kTile, r0 = blockIdx.y * kTile;
+  const int lr = t % kTile, lc = t / kTile;
+  const int c = c0 + lc, r = r0 + lr;
+  const bool node_ok = c <= ncols && r <= nrows;
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+
+  __shared__ int s_cs[kSeg][kTile + 2];  // cstart, cell rows rl..rh+1
+  __shared__ int s_sego[kSeg + 1];       // tile-list offset per column
+  __shared__ double s_Z[(kTile + 2) * (kTile + 2)];
+  __shared__ double s_p[kNS][kChunk];
+  __shared__ double s_red[6][kP2GThreads / 64];
+
+  const int ccl = c0 - 1 < 0 ? 0 : c0 - 1;
+  const int cch = c0 + kTile - 1 > ncols - 1 ? ncols - 1 : c0 + kTile - 1;
+  const int rl = r0 - 1 < 0 ? 0 : r0 - 1;
+  const int rh = r0 + kTile - 1 > nrows - 1 ? nrows - 1 : r0 + kTile - 1;
+  const int nseg = cch - ccl + 1, nr = rh - rl + 1;
+
+  for (int i = t; i < nseg * (nr + 1); i += kP2GThreads) {
+    const int s = i / (nr + 1), j = i % (nr + 1);
+    s_cs[s][j] = a.cstart[(ccl + s) * nrows + rl + j];
+  }
+  __syncthreads();
+  if (t == 0) {
+    int o = 0;
+    for (int s = 0; s < nseg; ++s) {
+      s_sego[s] = o;
+      o += s_cs[s][nr] - s_cs[s][0];
+    }
+    s_sego[nseg] = o;
+  }
+  __syncthreads();
+  const int L = s_sego[nseg];
+
+  double pm = 0, ppx = 0, ppy = 0;  // particle sums (owned cells)
+  double Mv = 0, mvx = 0, mvy = 0, FPx = 0, FPy = 0, Frx = 0, Fry = 0,
+         Fix = 0, Fiy = 0;
+
+  if (L > 0) {
+    // Z of nodes (c0-1 .. c0+16, r0-1 .. r0+16)
+    for (int i = t; i < (kTile + 2) * (kTile + 2); i += kP2GThreads) {
+      const int zc = c0 - 1 + i / (kTile + 2), zr = r0 - 1 + i % (kTile + 2);
+      s_Z[i] = (zc >= 0 && zc <= ncols && zr >= 0 && zr <= nrows)
+                   ? a.Z[zr + zc * (nrows + 1)]
+                   : 0.0;
+    }
+    // this node's four cells as tile-list ranges, ascending cell id:
+    // (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r) corner 0
+    int tb[4], te[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cc = c - 1 + (k >> 1), cr = r - 1 + (k & 1);
+      tb[k] = te[k] = 0;
+      if (node_ok && cc >= 0 && cc < ncols && cr >= 0 && cr < nrows) {
+        const int s = cc - ccl, j = cr - rl;
+        tb[k] = s_sego[s] + (s_cs[s][j] - s_cs[s][0]);
+        te[k] = s_sego[s] + (s_cs[s][j + 1] - s_cs[s][0]);
+      }
+    }
+    __syncthreads();
+
+    for (int base = 0; base < L; base += kChunk) {
+      const int q = base + t;
+      if (q < L) {
+        int s = 0;
+        while (s + 1 < nseg && s_sego[s + 1] <= q) ++s;
+        const int jp = s_cs[s][0] + (q - s_sego[s]);
+        const int ip = a.cptcls[jp];
+        const double x = a.p.f[PF_X][ip], y = a.p.f[PF_Y][ip];
+        const int cell = cell_of(x, y, nrows, ncols, hx, hy);
+        const int pr = cell % nrows, pc = cell / nrows;
+        const Local Lc = local_coords(x, y, pc, pr, hx, hy);
+        // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
+        const int zb = (pc - (c0 - 1)) * (kTile + 2) + (pr - (r0 - 1));
+        const double z0 = s_Z[zb], z1 = s_Z[zb + 1];
+        const double z2 = s_Z[zb + (kTile + 2)], z3 = s_Z[zb + (kTile + 3)];
+        double dzx = 0.0, dzy = 0.0;
+        dzx += shape_dx(Lc, 0, hx) * z0;
+        dzy += shape_dy(Lc, 0, hy) * z0;
+        dzx += shape_dx(Lc, 1, hx) * z1;
+        dzy += shape_dy(Lc, 1, hy) * z1;
+        dzx += shape_dx(Lc, 2, hx) * z2;
+        dzy += shape_dy(Lc, 2, hy) * z2;
+        dzx += shape_dx(Lc, 3, hx) * z3;
+        dzy += shape_dy(Lc, 3, hy) * z3;
+        const double hp = a.p.f[PF_HP][ip], H = a.p.f[PF_H][ip];
+        const double Vp = a.p.f[PF_VP][ip], Mp = a.p.f[PF_MP][ip];
+        // gravity along the slope with the well-balance factor,
+        // optim_par.cpp:450-453
+        const double wb = hp > 1e-10 ? 1.0 - (H / hp) : 1.0;
+        const double Fpx = -P.g * Vp * P.rho * dzx * wb;
+        const double Fpy = -P.g * Vp * P.rho * dzy * wb;
+        double mx, my;
+        if (a.mom_stored) {
+          mx = a.p.f[PF_MOMX][ip];
+          my = a.p.f[PF_MOMY][ip];
+        } else {
+          // mom_p = vpx * Mp (optim_par.cpp:600-601)
+          mx = a.p.f[PF_VPX][ip] * Mp;
+          my = a.p.f[PF_VPY][ip] * Mp;
+        }
+        s_p[0][t] = Lc.xi;
+        s_p[1][t] = Lc.eta;
+        s_p[2][t] = Mp;
+        s_p[3][t] = mx;
+        s_p[4][t] = my;
+        s_p[5][t] = Fpx;
+        s_p[6][t] = Fpy;
+        s_p[7][t] = a.p.f[PF_FRX][ip];
+        s_p[8][t] = a.p.f[PF_FRY][ip];
+        s_p[9][t] = a.p.f[PF_F11][ip];
+        s_p[10][t] = a.p.f[PF_F12][ip];
+        s_p[11][t] = a.p.f[PF_F22][ip];
+        s_p[12][t] = Vp;
+        const bool owned = pc >= c0 && pc < c0 + kTile && pr >= r0 && pr < r0 + kTile;
+        if (owned) {
+          pm += Mp;
+          ppx += mx;
+          ppy += my;
+          if (a.aux) {
+            a.p.f[PF_DZX][ip] = dzx;
+            a.p.f[PF_DZY][ip] = dzy;
+            a.p.f[PF_FPX][ip] = Fpx;
+            a.p.f[PF_FPY][ip] = Fpy;
+            a.p.f[PF_MOMX][ip] = mx;
+            a.p.f[PF_MOMY][ip] = my;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int lo = tb[k] > base ? tb[k] : base;
+        const int hi = te[k] < base + kChunk ? te[k] : base + kChunk;
+        const int corner = 3 - k;
+        for (int qq = lo; qq < hi; ++qq) {
+          const int i = qq - base;
+          Local Lq;
+          Lq.xi = s_p[0][i];
+          Lq.eta = s_p[1][i];
+          const double N = shape(Lq, corner);
+          Mv += N * s_p[2][i];
+          mvx += N * s_p[3][i];
+          mvy += N * s_p[4][i];
+          FPx += N * s_p[5][i];
+          FPy += N * s_p[6][i];
+          Frx += N * s_p[7][i];
+          Fry += N * s_p[8][i];
+          const double Nx = shape_dx(Lq, corner, hx);
+          const double Ny = shape_dy(Lq, corner, hy);
+          const double vp = s_p[12][i];
+          Fix += (Nx * s_p[9][i] + Ny * s_p[10][i]) * vp;
+          Fiy += (Nx * s_p[10][i] + Ny * s_p[11][i]) * vp;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  if (node_ok && (L > 0 || a.aux)) {
+    const int n = r + c * (nrows + 1);
+    const double dt = ctrl->dt;
+    // force balance and nodal velocities, optim_par.cpp:467-497 (D1)
+    const double Fex = FPx + Frx, Fey = FPy + Fry;
+    const double Ftx = Fex - Fix, Fty = Fey - Fiy;
+    double mx = dt * Ftx + mvx, my = dt * Fty + mvy;
+    double ax = Mv > 1.e-2 ? Ftx / Mv : 0.0;
+    double ay = Mv > 1.e-2 ? Fty / Mv : 0.0;
+    double vx = Mv > 1.e-2 ? mx / Mv : 0.0;
+    double vy = Mv > 1.e-2 ? my / Mv : 0.0;
+    double vxL = dt * ax + vx, vyL = dt * ay + vy;
+    // boundary conditions, optim_par.cpp:503-529 (nodes of the first/last
+    // cell column: x components; first/last cell row: y components)
+    if (P.bc) {
+      if (c == 0 || c == 1 || c == ncols - 1 || c == ncols) {
+        ax = 0.0; vx = 0.0; vxL = 0.0; mx = 0.0;
+      }
+      if (r == 0 || r == 1 || r == nrows - 1 || r == nrows) {
+        ay = 0.0; vy = 0.0; vyL = 0.0; my = 0.0;
+      }
+    }
+    a.n.f[NF_VVXL][n] = vxL;
+    a.n.f[NF_VVYL][n] = vyL;
+    if (a.aux) {
+      a.n.f[NF_AVX][n] = ax;
+      a.n.f[NF_AVY][n] = ay;
+      a.n.f[NF_MV][n] = Mv;
+      a.n.f[NF_MOMX][n] = mx;
+      a.n.f[NF_MOMY][n] = my;
+      a.n.f[NF_FEXTX][n] = Fex;
+      a.n.f[NF_FEXTY][n] = Fey;
+      a.n.f[NF_FINTX][n] = Fix;
+      a.n.f[NF_FINTY][n] = Fiy;
+      a.n.f[NF_FRX][n] = Frx;
+      a.n.f[NF_FRY][n] = Fry;
+      a.n.f[NF_FPX][n] = FPx;
+      a.n.f[NF_FPY][n] = FPy;
+      a.n.f[NF_FTOTX][n] = Ftx;
+      a.n.f[NF_FTOTY][n] = Fty;
+      a.n.f[NF_VVX][n] = vx;
+      a.n.f[NF_VVY][n] = vy;
+    }
+  }
+
+  if (a.partials) {
+    // deterministic block sums for the conservation check
+    double v[6] = {pm, ppx, ppy, node_ok ? Mv : 0.0, node_ok ? mvx : 0.0,
+                   node_ok ? mvy : 0.0};
+    const int lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double s = wave_sum(v[k]);
+      if (lane == 0) s_red[k][w] = s;
+    }
+    __syncthreads();
+    if (t < 6) {
+      double s = 0.0;
+      for (int ww = 0; ww < kP2GThreads / 64; ++ww) s += s_red[t][ww];
+      a.partials[bid * 6 + t] = s;
+    }
+  }
+}
+
+// err_mass, err_mom from the block partials, optim_par.cpp:416-425.
+__global__ void conservation_kernel(const double *partials, int nb,
+                                    Ctrl *ctrl) {
+  __shared__ double s[6][256];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += 256)
+    for (int k = 0; k < 6; ++k) v[k] += partials[b * 6 + k];
+  for (int k = 0; k < 6; ++k) s[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 6; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && ctrl->active) {
+    const double tm = s[0][0], tx = s[1][0], ty = s[2][0];
+    const double nm = s[3][0], nx = s[4][0], ny = s[5][0];
+    const double am = fabs(tx) > fabs(ty) ? fabs(tx) : fabs(ty);
+    const double norm = 1.0 > am ? 1.0 : am;
+    const double ex = fabs(tx - nx) / norm, ey = fabs(ty - ny) / norm;
+    ctrl->cons[0] = fabs(tm - nm) / tm;
+    ctrl->cons[1] = ex > ey ? ex : ey;
+  }
+}
+
+struct PartArgs {
+  KParams P;
+  int np;
+  PPtr p;
+  const int *key_in;  // step-start cell of each particle
+  int *key_out, *slot, *count;
+  const double *vvxL, *vvyL, *avx, *avy;
+  const double *Z;
+  Ctrl *ctrl;
+  int aux;
+};
+
+__global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
+  Ctrl *ctrl = a.ctrl;
+  const KParams &P = a.P;
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!ctrl->active) {
+    // step after the end of the run: carry the keys over unchanged
+    if (ip < a.np) a.key_out[ip] = a.key_in[ip];
+    return;
+  }
+  const bool valid = ip < a.np;
+  const double dt = ctrl->dt;
+  int newkey = 0;
+  double vmx = 0.0, vmy = 0.0, hmx = 0.0;
+  if (valid) {
+    const int nrows = P.nrows;
+    const double hx = P.hx, hy = P.hy;
+    const int cell = a.key_in[ip], r = cell % nrows, c = cell / nrows;
+    int nid[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) nid[k] = node_id(k, c, r, nrows);
+    double gx[4], gy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      gx[k] = a.vvxL[nid[k]];
+      gy[k] = a.vvyL[nid[k]];
+    }
+    const double x = a.p.f[PF_X][ip], y = a.p.f[PF_Y][ip];
+    const Local L0 = local_coords(x, y, c, r, hx, hy);
+    // g2p(vvxL, vvyL), optim_par.cpp:542
+    double vxL = 0.0, vyL = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double N = shape(L0, k);
+      vxL += N * gx[k];
+      vyL += N * gy[k];
+    }
+    // PIC velocity, advection and clamp, optim_par.cpp:548-564
+    const double vpx = vxL, vpy = vyL;
+    double xn = x + dt * vpx, yn = y + dt * vpy;
+    const double eps = 1e-10, Lx = P.ncols * P.hx, Ly = P.nrows * P.hy;
+    if (xn < eps) xn = eps;
+    if (xn > Lx - eps) xn = Lx - eps;
+    if (yn < eps) yn = eps;
+    if (yn > Ly - eps) yn = Ly - eps;
+    // g2pd(vvxL, vvyL) at the new position, step-start cell (D18),
+    // optim_par.cpp:578
+    const Local L1 = local_coords(xn, yn, c, r, hx, hy);
+    double vxdx = 0.0, vxdy = 0.0, vydx = 0.0, vydy = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double Nx = shape_dx(L1, k, hx), Ny = shape_dy(L1, k, hy);
+      vxdx += Nx * gx[k];
+      vxdy += Ny * gx[k];
+      vydx += Nx * gy[k];
+      vydy += Ny * gy[k];
+    }
+    // height update with floor, area, optim_par.cpp:595-607
+    const double div = vxdx + vydy;
+    double h = a.p.f[PF_HP][ip] / (1 + dt * div);
+    h = h < 1e-2 ? 1e-2 : h;
+    const double Vp = a.p.f[PF_VP][ip];
+    const double Ap = h > 1.e-2 ? Vp / h : 0.0;
+    // Voellmy basal friction, optim_par.cpp:613-626
+    double Fbx = 0.0, Fby = 0.0;
+    {
+      const double nv = sqrt(vpx * vpx + vpy * vpy);
+      if (P.fric > 0 && nv > 1e-10 && P.xi > 0) {
+        const double v2 = vpx * vpx + vpy * vpy;
+        Fbx = -P.fric * (P.rho * P.g * h * P.tan_fa + P.rho * P.g * v2 / P.xi) * vpx / nv;
+        Fby = -P.fric * (P.rho * P.g * h * P.tan_fa + P.rho * P.g * v2 / P.xi) * vpy / nv;
+      }
+    }
+    // Bingham stress, stress_tensor_t::operator(), optim_par.cpp:68-120
+    double F11, F12, F22;
+    {
+      const double A = 3. / 2., C = 65. / 32.;
+      const double mu = P.mu, tau_Y = P.tauy;
+      const double nrm = sqrt(vpx * vpx + vpy * vpy);
+      const double ALF = h > 1.e-3 ? (6. * mu * nrm) / ((h + 0.001) * tau_Y) : 0.0;
+      const double B = -114. / 32. - ALF;
+      const double Z1 = (-B + sqrt(B * B - 4. * A * C)) / (2. * A);
+      const double Z2 = (-B - sqrt(B * B - 4. * A * C)) / (2. * A);
+      const double ZZ = fabs(Z1 - .5) <= .5 ? Z1 : Z2;
+      const double D_xx = vxdx;
+      const double D_xy = 0.5 * (vxdy + vydx);
+      const double D_yy = vydy;
+      const double D_zx = h > 1.e-3 ? 0.5 * (3. / (2. + ZZ)) * (vpx / (h + 0.001)) : 0.0;
+      const double D_zy = h > 1.e-3 ? 0.5 * (3. / (2. + ZZ)) * (vpy / (h + 0.001)) : 0.0;
+      const double D_zz = -(vxdx + vydy);
+      const double invII = 0.5 * (D_xx * D_xx + D_yy * D_yy + D_zz * D_zz) +
+                           D_zx * D_zx + D_zy * D_zy + D_xy * D_xy;
+      const double sig_xx = invII != 0 ? (tau_Y / sqrt(invII) + 2. * mu) * D_xx : 0.0;
+      const double sig_xy = invII != 0 ? (tau_Y / sqrt(invII) + 2. * mu) * D_xy : 0.0;
+      const double sig_yy = invII != 0 ? (tau_Y / sqrt(invII) + 2. * mu) * D_yy : 0.0;
+      const double H = a.p.f[PF_H][ip];
+      const double h_corr = h > 1e-10 ? h - (H * H) / h : 0.0;
+      F11 = P.bing * sig_xx - .5 * P.rho * P.g * h_corr;
+      F12 = P.bing * sig_xy;
+      F22 = P.bing * sig_yy - .5 * P.rho * P.g * h_corr;
+    }
+    a.p.f[PF_X][ip] = xn;
+    a.p.f[PF_Y][ip] = yn;
+    a.p.f[PF_VPX][ip] = vpx;
+    a.p.f[PF_VPY][ip] = vpy;
+    a.p.f[PF_HP][ip] = h;
+    a.p.f[PF_F11][ip] = F11;
+    a.p.f[PF_F12][ip] = F12;
+    a.p.f[PF_F22][ip] = F22;
+    // friction force for the next P2G, optim_par.cpp:456-459
+    a.p.f[PF_FRX][ip] = Ap * Fbx;
+    a.p.f[PF_FRY][ip] = Ap * Fby;
+    if (a.aux) {
+      a.p.f[PF_AP][ip] = Ap;
+      a.p.f[PF_FBX][ip] = Fbx;
+      a.p.f[PF_FBY][ip] = Fby;
+      a.p.f[PF_VXDX][ip] = vxdx;
+      a.p.f[PF_VXDY][ip] = vxdy;
+      a.p.f[PF_VYDX][ip] = vydx;
+      a.p.f[PF_VYDY][ip] = vydy;
+      double ax = 0.0, ay = 0.0, zp = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double N = shape(L0, k);
+        ax += N * a.avx[nid[k]];
+        ay += N * a.avy[nid[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) zp += shape(L1, k) * a.Z[nid[k]];
+      a.p.f[PF_APX][ip] = ax;
+      a.p.f[PF_APY][ip] = ay;
+      a.p.f[PF_ZP][ip] = zp;
+      a.p.f[PF_HPZ][ip] = h + zp;
+      const double Mp = a.p.f[PF_MP][ip];
+      a.p.f[PF_MOMX][ip] = vpx * Mp;
+      a.p.f[PF_MOMY][ip] = vpy * Mp;
+    }
+    newkey = cell_of(xn, yn, P.nrows, P.ncols, hx, hy);
+    vmx = fabs(vpx);
+    vmy = fabs(vpy);
+    hmx = h > 0.0 ? h : 0.0;
+  }
+  const int sl = wave_slot(a.count, newkey, valid);
+  if (valid) {
+    a.key_out[ip] = newkey;
+    a.slot[ip] = sl;
+  }
+  // CFL maxima (non-negative doubles order like their bit patterns)
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    vmx = fmax(vmx, __shfl_xor(vmx, d));
+    vmy = fmax(vmy, __shfl_xor(vmy, d));
+    hmx = fmax(hmx, __shfl_xor(hmx, d));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    const int s = ctrl->it & 1;
+    unsigned long long *m = const_cast<unsigned long long *>(ctrl->cfl[s]);
+    atomicMax(&m[0], (unsigned long long)__double_as_longlong(vmx));
+    atomicMax(&m[1], (unsigned long long)__double_as_longlong(vmy));
+    atomicMax(&m[2], (unsigned long long)__double_as_longlong(hmx));
+  }
+}
+
+// Merge/split execution, merge_split_ops_cmes.h:258-427.
+struct ExecArgs {
+  int np, nrows, ncols;
+  double hx, hy;
+  PPtr in, out;
+  const int *act, *partner, *offset;
+};
+
+__global__ void ms_execute_kernel(ExecArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const int act = a.act[i];
+  const int o = a.offset[i];
+  const double Lx = a.ncols * a.hx, Ly = a.nrows * a.hy, eps = 1e-10;
+  if (act == MS_KEEP) {
+#pragma unroll 4
+    for (int k = 0; k < PF_COUNT; ++k) a.out.f[k][o] = a.in.f[k][i];
+    a.out.label[o] = a.in.label[i];
+    a.out.level[o] = a.in.level[i];
+  } else if (act == MS_SPLIT) {
+    const int o1 = o, o2 = o + 1;
+    double ux, uy;
+    const double exx = a.in.f[PF_VXDX][i], eyy = a.in.f[PF_VYDY][i];
+    const double exy = 0.5 * (a.in.f[PF_VXDY][i] + a.in.f[PF_VYDX][i]);
+    principal_stretch_dir(exx, eyy, exy, &ux, &uy);
+    const double r_i = sqrt(a.in.f[PF_AP][i]);
+    const double off = 0.25 * r_i;
+    const double cx = a.in.f[PF_X][i], cy = a.in.f[PF_Y][i];
+    double x1 = cx + off * ux, y1 = cy + off * uy;
+    double x2 = cx - off * ux, y2 = cy - off * uy;
+    x1 = x1 < eps ? eps : (x1 > Lx - eps ? Lx - eps : x1);
+    y1 = y1 < eps ? eps : (y1 > Ly - eps ? Ly - eps : y1);
+    x2 = x2 < eps ? eps : (x2 > Lx - eps ? Lx - eps : x2);
+    y2 = y2 < eps ? eps : (y2 > Ly - eps ? Ly - eps : y2);
+    for (int k = PF_VPX; k < PF_COUNT; ++k) {
+      const double val = a.in.f[k][i] / (pf_extensive(k) ? 2.0 : 1.0);
+      a.out.f[k][o1] = val;
+      a.out.f[k][o2] = val;
+    }
+    const double h = a.out.f[PF_HP][o1];
+    const double Ap1 = a.out.f[PF_VP][o1] / h;
+    a.out.f[PF_AP][o1] = Ap1;
+    a.out.f[PF_AP][o2] = a.out.f[PF_VP][o2] / h;
+    a.out.f[PF_X][o1] = x1; a.out.f[PF_Y][o1] = y1;
+    a.out.f[PF_X][o2] = x2; a.out.f[PF_Y][o2] = y2;
+    const double Mc = a.out.f[PF_MP][o1];
+    a.out.f[PF_MOMX][o1] = a.out.f[PF_MOMX][o2] = Mc * a.out.f[PF_VPX][o1];
+    a.out.f[PF_MOMY][o1] = a.out.f[PF_MOMY][o2] = Mc * a.out.f[PF_VPY][o1];
+    const double Fbx = a.out.f[PF_FBX][o1], Fby = a.out.f[PF_FBY][o1];
+    a.out.f[PF_FRX][o1] = a.out.f[PF_FRX][o2] = Ap1 * Fbx;
+    a.out.f[PF_FRY][o1] = a.out.f[PF_FRY][o2] = Ap1 * Fby;
+    a.out.label[o1] = a.out.label[o2] = -1;
+    a.out.level[o1] = a.out.level[o2] = a.in.level[i] - 1;
+  } else if (act == MS_PRIMARY) {
+    const int j = a.partner[i];
+    const double M1 = a.in.f[PF_MP][i], M2 = a.in.f[PF_MP][j], Mt = M1 + M2;
+    const double nx = (M1 * a.in.f[PF_X][i] + M2 * a.in.f[PF_X][j]) / Mt;
+    const double ny = (M1 * a.in.f[PF_Y][i] + M2 * a.in.f[PF_Y][j]) / Mt;
+    for (int k = PF_VPX; k < PF_COUNT; ++k)
+      a.out.f[k][o] = pf_extensive(k)
+                          ? (a.in.f[k][i] + a.in.f[k][j])
+                          : (M1 * a.in.f[k][i] + M2 * a.in.f[k][j]) / Mt;
+    a.out.f[PF_HP][o] = a.out.f[PF_VP][o] / a.out.f[PF_AP][o];
+    a.out.f[PF_X][o] = nx;
+    a.out.f[PF_Y][o] = ny;
+    const double Mp = a.out.f[PF_MP][o];
+    a.out.f[PF_MOMX][o] = Mp * a.out.f[PF_VPX][o];
+    a.out.f[PF_MOMY][o] = Mp * a.out.f[PF_VPY][o];
+    const double Ap = a.out.f[PF_AP][o];
+    a.out.f[PF_FRX][o] = Ap * a.out.f[PF_FBX][o];
+    a.out.f[PF_FRY][o] = Ap * a.out.f[PF_FBY][o];
+    a.out.label[o] = -1;
+    a.out.level[o] = a.in.level[i] + 1;
+  }
+}
+
+__global__ void weights_kernel(int np, const int *act, int *w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int a = act[i];
+  w[i] = a == MS_KEEP ? 1 : a == MS_SPLIT ? 2 : a == MS_PRIMARY ? 1 : 0;
+}
+
+__global__ void fill_f64_kernel(double *p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- host side
+struct TimedLaunch {
+  const char *name;
+  hipEvent_t a, b;
+};
+
+struct Sim {
+  amsc_params p{};
+  KParams kp{};
+  int np = 0, cap = 0, nn = 0, nc = 0;
+  hipStream_t st = nullptr;
+  PPtr buf[2]{};
+  int cur = 0;
+  int *key[2] = {nullptr, nullptr};
+  int kcur = 0;
+  int *slot = nullptr, *cptcls = nullptr;
+  Binning bin;
+  NPtr nod{};
+  double *Z = nullptr;
+  Ctrl *ctrl = nullptr;
+  Ctrl *ctrl_host = nullptr;  // pinned
+  double *partials = nullptr;
+  int nblk_x = 0, nblk_y = 0;
+  // merge/split workspace
+  int *act = nullptr, *partner = nullptr, *wts = nullptr, *offs = nullptr;
+  int *ms_cnt = nullptr;
+  MsParams msp{};
+  int ms_counts[2] = {0, 0};
+  // host mirrors
+  int it_host = 0;
+  bool finished = false;
+  bool aux_on = false;
+  bool diag = true;
+  bool mom_stored = false;
+  bool aux_valid = true;   // aux fields current for the present state
+  bool nodal_valid = false;
+  bool timing = false;
+  std::vector<TimedLaunch> launches;
+  std::vector<hipEvent_t> event_pool;
+  size_t event_used = 0;
+};
+
+namespace {
+
+int ev_get(Sim &s, hipEvent_t *e) {
+  if (s.event_used < s.event_pool.size()) {
+    *e = s.event_pool[s.event_used++];
+    return AMSC_OK;
+  }
+  AMSC_HIP(hipEventCreate(e));
+  s.event_pool.push_back(*e);
+  s.event_used++;
+  return AMSC_OK;
+}
+
+// Bracket a launch with events when timing is on.
+struct Timer {
+  Sim &s;
+  TimedLaunch tl{};
+  bool on;
+  Timer(Sim &sim, const char *name) : s(sim), on(sim.timing) {
+    if (on) {
+      tl.name = name;
+      if (ev_get(s, &tl.a) || ev_get(s, &tl.b)) {
+        on = false;
+        return;
+      }
+      (void)hipEventRecord(tl.a, s.st);
+    }
+  }
+  ~Timer() {
+    if (on) {
+      (void)hipEventRecord(tl.b, s.st);
+      s.launches.push_back(tl);
+    }
+  }
+};
+
+PPtr &cur_buf(Sim &s) { return s.buf[s.cur]; }
+
+int alloc_ppt(PPtr &p, int cap) {
+  for (int k = 0; k < PF_COUNT; ++k)
+    if (hipMalloc(&p.f[k], sizeof(double) * cap) != hipSuccess)
+      return fail(AMSC_ERR_NOMEM, "particle array allocation failed");
+  if (hipMalloc(&p.label, sizeof(int) * cap) != hipSuccess ||
+      hipMalloc(&p.level, sizeof(int) * cap) != hipSuccess)
+    return fail(AMSC_ERR_NOMEM, "particle array allocation failed");
+  return AMSC_OK;
+}
+
+void free_ppt(PPtr &p) {
+  for (int k = 0; k < PF_COUNT; ++k)
+    if (p.f[k]) (void)hipFree(p.f[k]);
+  if (p.label) (void)hipFree(p.label);
+  if (p.level) (void)hipFree(p.level);
+  p = PPtr();
+}
+
+void sim_free(Sim *s) {
+  if (!s) return;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  free_ppt(s->buf[0]);
+  free_ppt(s->buf[1]);
+  for (int i = 0; i < 2; ++i)
+    if (s->key[i]) (void)hipFree(s->key[i]);
+  int *ip[] = {s->slot, s->cptcls, s->act, s->partner, s->wts, s->offs, s->ms_cnt};
+  for (int *q : ip)
+    if (q) (void)hipFree(q);
+  binning_free(s->bin);
+  for (int k = 0; k < NF_COUNT; ++k)
+    if (s->nod.f[k]) (void)hipFree(s->nod.f[k]);
+  if (s->Z) (void)hipFree(s->Z);
+  if (s->ctrl) (void)hipFree(s->ctrl);
+  if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
+  if (s->partials) (void)hipFree(s->partials);
+  for (auto e : s->event_pool) (void)hipEventDestroy(e);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+}
+
+KParams kparams_from(const amsc_params &p) {
+  KParams k;
+  k.nrows = p.nrows;
+  k.ncols = p.ncols;
+  k.hx = p.hx;
+  k.hy = p.hy;
+  k.g = p.g;
+  k.rho = p.rho;
+  k.mu = p.mu;
+  k.tauy = p.tauy;
+  k.xi = p.xi;
+  // fric_ang = phi * M_PI / 180 (optim_par.cpp:193), tan on the host
+  k.tan_fa = std::tan(p.phi * M_PI / 180.);
+  k.bing = p.bingham_on;
+  k.fric = p.friction_on;
+  k.cfl = p.cfl;
+  k.eq_level = p.eq_level;
+  k.dt_fixed = p.dt_fixed;
+  k.T = p.T;
+  k.bc = p.bc_flag;
+  k.nsteps = p.nsteps;
+  k.max_steps = p.max_steps;
+  return k;
+}
+
+// Bin the current particle set into key[kcur], cptcls, cstart.
+int rebin(Sim &s) {
+  PPtr &p = cur_buf(s);
+  {
+    Timer tm(s, "key_count");
+    if (int rc = launch_key_count(s.np, p.f[PF_X], p.f[PF_Y], s.p.nrows,
+                                  s.p.ncols, s.p.hx, s.p.hy, s.key[s.kcur],
+                                  s.slot, s.bin.count, nullptr, s.st))
+      return rc;
+  }
+  Timer tm(s, "binning");
+  return launch_finish_binning(s.bin, s.np, s.key[s.kcur], s.slot, s.cptcls,
+                               nullptr, s.st);
+}
+
+int read_ctrl(Sim &s) {
+  AMSC_HIP(hipMemcpyAsync(s.ctrl_host, s.ctrl, sizeof(Ctrl),
+                          hipMemcpyDeviceToHost, s.st));
+  AMSC_HIP(hipStreamSynchronize(s.st));
+  return AMSC_OK;
+}
+
+int merge_split(Sim &s) {
+  PPtr &p = cur_buf(s);
+  MsInputs in = {p.f[PF_X],    p.f[PF_Y],    p.f[PF_AP],   p.f[PF_HP],
+                 p.f[PF_VXDX], p.f[PF_VXDY], p.f[PF_VYDX], p.f[PF_VYDY],
+                 p.f[PF_VPX],  p.f[PF_VPY],  p.level};
+  int counts[2];
+  {
+    Timer tm(s, "ms_decide");
+    if (int rc = ms_decide(s.msp, s.np, s.nc, &s.ctrl->dt, 0.0, in,
+                           s.bin.cstart, s.cptcls, s.act, s.partner,
+                           s.ms_cnt, counts, s.st))
+      return rc;
+  }
+  s.ms_counts[0] = counts[0];
+  s.ms_counts[1] = counts[1];
+  if (counts[0] == 0 && counts[1] == 0) return AMSC_OK;  // no-op
+  int nnew = 0;
+  {
+    Timer tm(s, "ms_execute");
+    weights_kernel<<<grid_for(s.np, 256), 256, 0, s.st>>>(s.np, s.act, s.wts);
+    AMSC_LAUNCH_CHECK();
+    if (int rc = exclusive_scan(s.wts, s.offs, s.np, false, s.bin.scan, s.st,
+                                &nnew))
+      return rc;
+    if (nnew > s.cap)
+      return fail(AMSC_ERR_CAPACITY,
+                  "merge/split needs " + std::to_string(nnew) +
+                      " particles, capacity is " + std::to_string(s.cap));
+    ExecArgs ea;
+    ea.np = s.np;
+    ea.nrows = s.p.nrows;
+    ea.ncols = s.p.ncols;
+    ea.hx = s.p.hx;
+    ea.hy = s.p.hy;
+    ea.in = s.buf[s.cur];
+    ea.out = s.buf[1 - s.cur];
+    ea.act = s.act;
+    ea.partner = s.partner;
+    ea.offset = s.offs;
+    ms_execute_kernel<<<grid_for(s.np, 256), 256, 0, s.st>>>(ea);
+    AMSC_LAUNCH_CHECK();
+  }
+  s.cur = 1 - s.cur;
+  s.np = nnew;
+  return rebin(s);
+}
+
+int launch_step(Sim &s) {
+  const bool ms_step = s.p.ms_on && s.p.ms_interval > 0 &&
+                       ((s.it_host + 1) % s.p.ms_interval == 0);
+  const bool ms_next = s.p.ms_on && s.p.ms_interval > 0 &&
+                       ((s.it_host + 2) % s.p.ms_interval == 0);
+  const bool aux = s.aux_on || ms_next;
+  {
+    Timer tm(s, "prologue");
+    prologue_kernel<<<1, 1, 0, s.st>>>(s.kp, s.ctrl);
+    AMSC_LAUNCH_CHECK();
+  }
+  if (ms_step) {
+    if (int rc = read_ctrl(s)) return rc;
+    if (!s.ctrl_host->active) {
+      s.finished = true;
+      return AMSC_OK;
+    }
+    if (int rc = merge_split(s)) return rc;
+  }
+  PPtr &p = cur_buf(s);
+  {
+    Timer tm(s, "p2g_nodal");
+    P2GArgs a;
+    a.P = s.kp;
+    a.cstart = s.bin.cstart;
+    a.cptcls = s.cptcls;
+    a.p = p;
+    a.Z = s.Z;
+    a.ctrl = s.ctrl;
+    a.n = s.nod;
+    a.partials = s.diag ? s.partials : nullptr;
+    a.aux = aux ? 1 : 0;
+    a.mom_stored = s.mom_stored ? 1 : 0;
+    dim3 grid(s.nblk_x, s.nblk_y);
+    p2g_nodal_kernel<<<grid, kP2GThreads, 0, s.st>>>(a);
+    AMSC_LAUNCH_CHECK();
+  }
+  if (s.diag) {
+    Timer tm(s, "conservation");
+    conservation_kernel<<<1, 256, 0, s.st>>>(s.partials, s.nblk_x * s.nblk_y,
+                                             s.ctrl);
+    AMSC_LAUNCH_CHECK();
+  }
+  s.mom_stored = false;
+  {
+    Timer tm(s, "particle");
+    PartArgs a;
+    a.P = s.kp;
+    a.np = s.np;
+    a.p = p;
+    a.key_in = s.key[s.kcur];
+    a.key_out = s.key[1 - s.kcur];
+    a.slot = s.slot;
+    a.count = s.bin.count;
+    a.vvxL = s.nod.f[NF_VVXL];
+    a.vvyL = s.nod.f[NF_VVYL];
+    a.avx = s.nod.f[NF_AVX];
+    a.avy = s.nod.f[NF_AVY];
+    a.Z = s.Z;
+    a.ctrl = s.ctrl;
+    a.aux = aux ? 1 : 0;
+    if (s.np > 0) {
+      particle_kernel<<<grid_for(s.np, 256), 256, 0, s.st>>>(a);
+      AMSC_LAUNCH_CHECK();
+    }
+  }
+  s.kcur = 1 - s.kcur;
+  {
+    Timer tm(s, "binning");
+    if (int rc = launch_finish_binning(s.bin, s.np, s.key[s.kcur], s.slot,
+                                       s.cptcls, &s.ctrl->active, s.st))
+      return rc;
+  }
+  s.aux_valid = aux;
+  s.nodal_valid = aux;
+  s.it_host++;
+  if (s.p.dt_fixed > 0.0 && s.it_host >= s.p.nsteps) s.finished = true;
+  if (s.p.dt_fixed <= 0.0 && s.p.max_steps > 0 && s.it_host >= s.p.max_steps)
+    s.finished = true;
+  return AMSC_OK;
+}
+
+int find_pf(const char *name) {
+  for (int k = 0; k < PF_COUNT; ++k)
+    if (!std::strcmp(name, kPFName[k])) return k;
+  return -1;
+}
+int find_nf(const char *name) {
+  for (int k = 0; k < NF_COUNT; ++k)
+    if (!std::strcmp(name, kNFName[k])) return k;
+  return -1;
+}
+
+}  // namespace
+
+}  // namespace amsc
+
+using namespace amsc;
+
+struct amsc_sim : amsc::Sim {};
+
+extern "C" {
+
+int amsc_sim_create(const amsc_params *p, int np, const double *x,
+                    const double *y, const double *Mp, const double *Ap,
+                    const double *vpx, const double *vpy,
+                    const double *mom_px, const double *mom_py,
+                    const double *hp, const double *Vp, const double *Z,
+                    int capacity, amsc_sim **out) {
+  if (!p || !out || np < 0 || p->nrows <= 0 || p->ncols <= 0 ||
+      !(p->hx > 0) || !(p->hy > 0))
+    return fail(AMSC_ERR_INVALID, "amsc_sim_create: bad parameters");
+  if ((long long)(p->nrows + 1) * (p->ncols + 1) >= (1ll << 31))
+    return fail(AMSC_ERR_INVALID, "amsc_sim_create: grid too large");
+  if (np > 0 && (!x || !y || !Mp || !Ap || !vpx || !vpy || !mom_px ||
+                 !mom_py || !hp || !Vp))
+    return fail(AMSC_ERR_INVALID, "amsc_sim_create: NULL particle array");
+  if (!Z) return fail(AMSC_ERR_INVALID, "amsc_sim_create: NULL Z");
+  *out = nullptr;
+  amsc_sim *s = new amsc_sim();
+  s->p = *p;
+  s->kp = kparams_from(*p);
+  s->msp = ms_params_from(*p);
+  s->np = np;
+  s->cap = capacity > 0 ? capacity : 2 * (np > 0 ? np : 1);
+  if (s->cap < np) s->cap = np;
+  s->nn = (p->nrows + 1) * (p->ncols + 1);
+  s->nc = p->nrows * p->ncols;
+  s->nblk_x = (p->ncols + 1 + kTile - 1) / kTile;
+  s->nblk_y = (p->nrows + 1 + kTile - 1) / kTile;
+  auto bail = [&](int rc) {
+    sim_free(s);
+    return rc;
+  };
+  if (hipStreamCreate(&s->st) != hipSuccess)
+    return bail(fail(AMSC_ERR_HIP, "stream creation failed"));
+  const int cap = s->cap;
+  for (int b = 0; b < 2; ++b)
+    if (int rc = alloc_ppt(s->buf[b], cap)) return bail(rc);
+  bool ok = true;
+  ok &= hipMalloc(&s->key[0], sizeof(int) * cap) == hipSuccess;
+  ok &= hipMalloc(&s->key[1], sizeof(int) * cap) == hipSuccess;
+  ok &= hipMalloc(&s->slot, sizeof(int) * cap) == hipSuccess;
+  ok &= hipMalloc(&s->cptcls, sizeof(int) * cap) == hipSuccess;
+  ok &= hipMalloc(&s->act, sizeof(int) * cap) == hipSuccess;
+  ok &= hipMalloc(&s->partner, sizeof(int) * cap) == hipSuccess;
+  ok &= hipMalloc(&s->wts, sizeof(int) * cap) == hipSuccess;
+  ok &= hipMalloc(&s->offs, sizeof(int) * (cap + 1)) == hipSuccess;
+  ok &= hipMalloc(&s->ms_cnt, sizeof(int) * 4) == hipSuccess;
+  for (int k = 0; k < NF_COUNT; ++k)
+    ok &= hipMalloc(&s->nod.f[k], sizeof(double) * s->nn) == hipSuccess;
+  ok &= hipMalloc(&s->Z, sizeof(double) * s->nn) == hipSuccess;
+  ok &= hipMalloc(&s->ctrl, sizeof(Ctrl)) == hipSuccess;
+  ok &= hipHostMalloc(&s->ctrl_host, sizeof(Ctrl)) == hipSuccess;
+  ok &= hipMalloc(&s->partials,
+                  sizeof(double) * 6 * s->nblk_x * s->nblk_y) == hipSuccess;
+  if (!ok) return bail(fail(AMSC_ERR_NOMEM, "amsc_sim_create: allocation failed"));
+  if (int rc = binning_alloc(s->bin, p->nrows, p->ncols)) return bail(rc);
+
+  hipStream_t st = s->st;
+  PPtr &b = s->buf[0];
+  const size_t bytes = sizeof(double) * (size_t)np;
+  const double *src[] = {x, y, vpx, vpy, hp, Vp, Mp, Ap};
+  const int dst[] = {PF_X, PF_Y, PF_VPX, PF_VPY, PF_HP, PF_VP, PF_MP, PF_AP};
+  if (np > 0)
+    for (int k = 0; k < 8; ++k)
+      if (hipMemcpyAsync(b.f[dst[k]], src[k], bytes, hipMemcpyHostToDevice,
+                         st) != hipSuccess)
+        return bail(fail(AMSC_ERR_HIP, "upload failed"));
+  if (hipMemcpyAsync(s->Z, Z, sizeof(double) * s->nn, hipMemcpyHostToDevice,
+                     st) != hipSuccess)
+    return bail(fail(AMSC_ERR_HIP, "upload failed"));
+  for (int k = 0; k < NF_COUNT; ++k)
+    fill_f64_kernel<<<grid_for(s->nn, 256), 256, 0, st>>>(s->nod.f[k], s->nn, 0.0);
+  // The first step's P2G uses the DATA momentum (optim_par.cpp:151-152);
+  // afterwards mom_p == vpx * Mp exactly, which P2G recomputes.
+  for (int i = 0; i < np && !s->mom_stored; ++i)
+    if (!(mom_px[i] == vpx[i] * Mp[i]) || !(mom_py[i] == vpy[i] * Mp[i]) ||
+        std::signbit(mom_px[i]) != std::signbit(vpx[i] * Mp[i]) ||
+        std::signbit(mom_py[i]) != std::signbit(vpy[i] * Mp[i]))
+      s->mom_stored = true;
+  double *mtmp[2] = {nullptr, nullptr};
+  if (np > 0) {
+    ok &= hipMalloc(&mtmp[0], bytes) == hipSuccess;
+    ok &= hipMalloc(&mtmp[1], bytes) == hipSuccess;
+    if (!ok) return bail(fail(AMSC_ERR_NOMEM, "allocation failed"));
+    (void)hipMemcpyAsync(mtmp[0], mom_px, bytes, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyAsync(mtmp[1], mom_py, bytes, hipMemcpyHostToDevice, st);
+  }
+  Ctrl c0;
+  std::memset(&c0, 0, sizeof(c0));
+  c0.dt = 1.0e-3;  // gpu_offload.cpp:255
+  c0.active = 1;
+  if (hipMemcpyAsync(s->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, st) !=
+      hipSuccess)
+    return bail(fail(AMSC_ERR_HIP, "upload failed"));
+  if (int rc = rebin(*s)) return bail(rc);
+  if (np > 0) {
+    init_kernel<<<grid_for(np, 256), 256, 0, st>>>(np, s->kp, b, s->key[s->kcur],
+                                                   s->Z, mtmp[0], mtmp[1]);
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    return bail(fail(AMSC_ERR_HIP, "initialisation failed"));
+  if (mtmp[0]) (void)hipFree(mtmp[0]);
+  if (mtmp[1]) (void)hipFree(mtmp[1]);
+  s->launches.clear();
+  s->event_used = 0;
+  *out = s;
+  return AMSC_OK;
+}
+
+int amsc_sim_destroy(amsc_sim *s) {
+  sim_free(s);
+  return AMSC_OK;
+}
+
+int amsc_sim_step(amsc_sim *s, int nsteps, int *done) {
+  if (!s || nsteps < 0) return fail(AMSC_ERR_INVALID, "amsc_sim_step: bad arguments");
+  if (int rc = read_ctrl(*s)) return rc;
+  const int it0 = s->ctrl_host->it;
+  s->it_host = it0;
+  if (!s->ctrl_host->active) s->finished = true;
+  for (int k = 0; k < nsteps && !s->finished; ++k)
+    if (int rc = launch_step(*s)) return rc;
+  if (int rc = read_ctrl(*s)) return rc;
+  if (!s->ctrl_host->active) s->finished = true;
+  s->it_host = s->ctrl_host->it;
+  if (done) *done = s->ctrl_host->it - it0;
+  return AMSC_OK;
+}
+
+int amsc_sim_set_aux(amsc_sim *s, int on) {
+  if (!s) return fail(AMSC_ERR_INVALID, "NULL sim");
+  s->aux_on = on != 0;
+  return AMSC_OK;
+}
+
+int amsc_sim_set_timing(amsc_sim *s, int on) {
+  if (!s) return fail(AMSC_ERR_INVALID, "NULL sim");
+  s->timing = on != 0;
+  return AMSC_OK;
+}
+
+int amsc_sim_kernel_times(amsc_sim *s, const char **names, double *ms,
+                          int *count, int cap, int *n) {
+  if (!s || !n) return fail(AMSC_ERR_INVALID, "amsc_sim_kernel_times: bad arguments");
+  AMSC_HIP(hipStreamSynchronize(s->st));
+  std::vector<const char *> nm;
+  std::vector<double> tot;
+  std::vector<int> cnt;
+  for (auto &l : s->launches) {
+    float e = 0.f;
+    AMSC_HIP(hipEventElapsedTime(&e, l.a, l.b));
+    size_t i = 0;
+    while (i < nm.size() && std::strcmp(nm[i], l.name)) ++i;
+    if (i == nm.size()) {
+      nm.push_back(l.name);
+      tot.push_back(0.0);
+      cnt.push_back(0);
+    }
+    tot[i] += e;
+    cnt[i] += 1;
+  }
+  *n = (int)nm.size();
+  for (int i = 0; i < (int)nm.size() && i < cap; ++i) {
+    if (names) names[i] = nm[i];
+    if (ms) ms[i] = tot[i];
+    if (count) count[i] = cnt[i];
+  }
+  return AMSC_OK;
+}
+
+int amsc_sim_reset_kernel_times(amsc_sim *s) {
+  if (!s) return fail(AMSC_ERR_INVALID, "NULL sim");
+  AMSC_HIP(hipStreamSynchronize(s->st));
+  s->launches.clear();
+  s->event_used = 0;
+  return AMSC_OK;
+}
+
+int amsc_sim_num_particles(amsc_sim *s, int *np) {
+  if (!s || !np) return fail(AMSC_ERR_INVALID, "bad arguments");
+  *np = s->np;
+  return AMSC_OK;
+}
+
+int amsc_sim_state(amsc_sim *s, int *it, double *t, double *dt) {
+  if (!s) return fail(AMSC_ERR_INVALID, "NULL sim");
+  if (int rc = read_ctrl(*s)) return rc;
+  if (it) *it = s->ctrl_host->it;
+  if (t) *t = s->ctrl_host->t;
+  if (dt) *dt = s->ctrl_host->dt;
+  return AMSC_OK;
+}
+
+int amsc_sim_get_field(amsc_sim *s, const char *name, double *out, int n) {
+  if (!s || !name || !out) return fail(AMSC_ERR_INVALID, "bad arguments");
+  const double *src = nullptr;
+  int len = 0;
+  const int k = find_pf(name);
+  if (k >= 0) {
+    if (k >= PF_CORE && !s->aux_valid)
+      return fail(AMSC_ERR_STALE, std::string("field '") + name +
+                                      "' was not written by the last step "
+                                      "(enable amsc_sim_set_aux)");
+    src = s->buf[s->cur].f[k];
+    len = s->np;
+  } else if (!std::strcmp(name, "Z")) {
+    src = s->Z;
+    len = s->nn;
+  } else if (!std::strcmp(name, "vpxL") || !std::strcmp(name, "vpyL")) {
+    src = s->buf[s->cur].f[name[2] == 'x' ? PF_VPX : PF_VPY];
+    len = s->np;
+  } else if (!std::strcmp(name, "F_21")) {
+    src = s->buf[s->cur].f[PF_F12];  // F_21 == F_12 (optim_par.cpp:116-117)
+    len = s->np;
+  } else {
+    const int j = find_nf(name);
+    if (j < 0) return fail(AMSC_ERR_INVALID, std::string("unknown field ") + name);
+    if (j >= NF_AVX && !s->nodal_valid)
+      return fail(AMSC_ERR_STALE, std::string("nodal field '") + name +
+                                      "' was not written by the last step");
+    src = s->nod.f[j];
+    len = s->nn;
+  }
+  if (n < len) return fail(AMSC_ERR_INVALID, "output buffer too small");
+  AMSC_HIP(hipMemcpyAsync(out, src, sizeof(double) * (size_t)len,
+                          hipMemcpyDeviceToHost, s->st));
+  AMSC_HIP(hipStreamSynchronize(s->st));
+  return AMSC_OK;
+}
+
+int amsc_sim_get_field_int(amsc_sim *s, const char *name, int *out, int n) {
+  if (!s || !name || !out) return fail(AMSC_ERR_INVALID, "bad arguments");
+  const int *src = nullptr;
+  int len = s->np;
+  if (!std::strcmp(name, "label")) src = s->buf[s->cur].label;
+  else if (!std::strcmp(name, "level")) src = s->buf[s->cur].level;
+  else if (!std::strcmp(name, "p2g")) src = s->key[s->kcur];
+  else if (!std::strcmp(name, "cptcls")) src = s->cptcls;
+  else if (!std::strcmp(name, "cstart")) {
+    src = s->bin.cstart;
+    len = s->nc + 1;
+  } else return fail(AMSC_ERR_INVALID, std::string("unknown field ") + name);
+  if (n < len) return fail(AMSC_ERR_INVALID, "output buffer too small");
+  AMSC_HIP(hipMemcpyAsync(out, src, sizeof(int) * (size_t)len,
+                          hipMemcpyDeviceToHost, s->st));
+  AMSC_HIP(hipStreamSynchronize(s->st));
+  return AMSC_OK;
+}
+
+int amsc_sim_set_field(amsc_sim *s, const char *name, const double *in, int n) {
+  if (!s || !name || !in) return fail(AMSC_ERR_INVALID, "bad arguments");
+  const int k = find_pf(name);
+  if (k < 0) return fail(AMSC_ERR_INVALID, std::string("unknown field ") + name);
+  if (n != s->np) return fail(AMSC_ERR_INVALID, "length must equal num_particles");
+  AMSC_HIP(hipMemcpyAsync(s->buf[s->cur].f[k], in, sizeof(double) * (size_t)n,
+                          hipMemcpyHostToDevice, s->st));
+  AMSC_HIP(hipStreamSynchronize(s->st));
+  if (k == PF_X || k == PF_Y) return rebin(*s);
+  return AMSC_OK;
+}
+
+int amsc_sim_conservation(amsc_sim *s, double out[3]) {
+  if (!s || !out) return fail(AMSC_ERR_INVALID, "bad arguments");
+  if (int rc = read_ctrl(*s)) return rc;
+  out[0] = s->ctrl_host->cons[0];
+  out[1] = s->ctrl_host->cons[1];
+  out[2] = s->ctrl_host->t_start;
+  return AMSC_OK;
+}
+
+int amsc_sim_ms_counts(amsc_sim *s, int out[2]) {
+  if (!s || !out) return fail(AMSC_ERR_INVALID, "bad arguments");
+  out[0] = s->ms_counts[0];
+  out[1] = s->ms_counts[1];
+  return AMSC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+/*
+ * amsc_mpm.h -- C ABI of the MI355X (gfx950) depth-averaged MPM time step.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every function returns
+ * AMSC_OK (0) or an AMSC_ERR_* code; amsc_last_error() gives the message.
+ * Launches are stream-ordered on the stream set with amsc_set_stream
+ * (default: the null stream) and do not synchronise unless stated.
+ *
+ * Three groups of entry points:
+ *   1. the 18 step kernels of the reference's gpu_kernels.h, same parameter
+ *      lists (device pointers), same results;
+ *   2. device binning (replaces quadgrid's host init_particle_mesh + the
+ *      per-step host CSR rebuild) and merge/split decisions on the device;
+ *   3. amsc_sim_*: a device-resident time loop (the gpu_offload.cpp driver's
+ *      job) built on fused kernels, FP64 with optim_par semantics.
+ */
+#ifndef AMSC_MPM_H
+#define AMSC_MPM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMSC_OK 0
+#define AMSC_ERR_INVALID 1   /* bad argument (size, pointer, layout)      */
+#define AMSC_ERR_HIP 2       /* a HIP runtime call failed                  */
+#define AMSC_ERR_NOMEM 3     /* device allocation failed                   */
+#define AMSC_ERR_CAPACITY 4  /* particle capacity exceeded                 */
+#define AMSC_ERR_STALE 5     /* field not materialised by the last step    */
+
+const char *amsc_last_error(void);
+/* hipStream_t for subsequent launches (NULL = null stream). */
+int amsc_set_stream(void *stream);
+int amsc_synchronize(void);
+
+/* ------------------------------------------------------------------ 1. */
+/* Same parameters and results as gpu_kernels:: in the reference
+ * (main/gpu_kernels.h line cited per entry point).  All double and int
+ * pointers are device pointers, except `coloff` (int[5], host memory, as in
+ * the reference, which dereferences it on the host: gpu_kernels.cpp:105-106)
+ * and the diagnostic *_out pointers, which the reference never writes
+ * (gpu_kernels.cpp:450-453,539-545) and which are ignored here too.
+ * Scatter entry points sum, for every node, the contributions of its (up to
+ * 4) cells in colour order, each cell's particles in cptcls order, added to
+ * the node's current value: the reference's order.  They require ccidx to
+ * list every cell (coloff[4] == nrows * ncols, as quadgrid's colouring does,
+ * gpu_offload.cpp:258). */
+
+/* gpu_kernels.h:7-18 */
+int amsc_reset_nodal_arrays(int nn, double *d_Mv, double *d_mom_vx,
+                            double *d_mom_vy, double *d_F_ext_vx,
+                            double *d_F_ext_vy, double *d_F_int_vx,
+                            double *d_F_int_vy, double *d_Ftot_vx,
+                            double *d_Ftot_vy, double *d_avx, double *d_avy,
+                            double *d_vvx, double *d_vvy, double *d_vvxL,
+                            double *d_vvyL, double *d_Fric_x, double *d_Fric_y,
+                            double *d_FPxv, double *d_FPyv, double *d_HV,
+                            double *d_pv_rho, double *d_pv_vvx,
+                            double *d_pv_vvy, double *d_pv_avx,
+                            double *d_pv_avy);
+/* gpu_kernels.h:23-28 */
+int amsc_scatter_mass_momentum(int np, int nrows, double hx, double hy,
+                               const int *cstart, const int *cptcls,
+                               const int *ccidx, const int *coloff,
+                               const double *d_x, const double *d_y,
+                               const double *d_Mp, const double *d_mom_px,
+                               const double *d_mom_py, double *d_Mv,
+                               double *d_mom_vx, double *d_mom_vy);
+/* gpu_kernels.h:33-37 */
+int amsc_gradZ_to_particles(int np, int nrows, double hx, double hy,
+                            const double *d_x, const double *d_y,
+                            const int *d_p2g, const double *d_Z,
+                            double *d_dZxp, double *d_dZyp);
+/* gpu_kernels.h:42-46 */
+int amsc_gravity_force(int np, double g_c, const double *d_hp,
+                       const double *d_H, const double *d_Mp,
+                       const double *d_dZxp, const double *d_dZyp,
+                       double *d_Fpx, double *d_Fpy);
+/* gpu_kernels.h:51-54 */
+int amsc_friction_particle_force(int np, const double *d_Ap,
+                                 const double *d_Fb_x, const double *d_Fb_y,
+                                 double *d_Fric_px, double *d_Fric_py);
+/* gpu_kernels.h:59-64 */
+int amsc_scatter_gravity_force(int np, int nrows, double hx, double hy,
+                               const int *cstart, const int *cptcls,
+                               const int *ccidx, const int *coloff,
+                               const double *d_x, const double *d_y,
+                               const double *d_Fpx, const double *d_Fpy,
+                               double *d_FPxv, double *d_FPyv);
+/* gpu_kernels.h:69-74 */
+int amsc_scatter_friction_force(int np, int nrows, double hx, double hy,
+                                const int *cstart, const int *cptcls,
+                                const int *ccidx, const int *coloff,
+                                const double *d_x, const double *d_y,
+                                const double *d_Fric_px,
+                                const double *d_Fric_py, double *d_Fric_x,
+                                double *d_Fric_y);
+/* gpu_kernels.h:79-83 */
+int amsc_compute_F_ext(int nn, const double *d_FPxv, const double *d_FPyv,
+                       const double *d_Fric_x, const double *d_Fric_y,
+                       double *d_F_ext_vx, double *d_F_ext_vy);
+/* gpu_kernels.h:88-94 */
+int amsc_scatter_internal_forces(int np, int nrows, double hx, double hy,
+                                 const int *cstart, const int *cptcls,
+                                 const int *ccidx, const int *coloff,
+                                 const double *d_x, const double *d_y,
+                                 const double *d_Vp, const double *d_F11,
+                                 const double *d_F12, const double *d_F21,
+                                 const double *d_F22, double *d_F_int_vx,
+                                 double *d_F_int_vy);
+/* gpu_kernels.h:99-104 */
+int amsc_nodal_force_and_momentum(int nn, double dt, const double *d_F_ext_vx,
+                                  const double *d_F_ext_vy,
+                                  const double *d_F_int_vx,
+                                  const double *d_F_int_vy, double *d_Ftot_vx,
+                                  double *d_Ftot_vy, double *d_mom_vx,
+                                  double *d_mom_vy);
+/* gpu_kernels.h:109-115 (bool bc_flag passed as int) */
+int amsc_nodal_velocity_and_bc(int nn, int nrows, int ncols, double dt,
+                               int bc_flag, const double *d_Mv,
+                               const double *d_Ftot_vx,
+                               const double *d_Ftot_vy, double *d_mom_vx,
+                               double *d_mom_vy, double *d_avx, double *d_avy,
+                               double *d_vvx, double *d_vvy, double *d_vvxL,
+                               double *d_vvyL);
+/* gpu_kernels.h:121-131 */
+int amsc_g2p_velocity_and_advect(int np, int nrows, int ncols, double hx,
+                                 double hy, double dt, const int *d_p2g,
+                                 const double *d_vvxL, const double *d_vvyL,
+                                 const double *d_avx, const double *d_avy,
+                                 double *d_x, double *d_y, double *d_vpx,
+                                 double *d_vpy, double *d_apx, double *d_apy,
+                                 double *d_mom_px, double *d_mom_py,
+                                 double *d_vpxL, double *d_vpyL,
+                                 const double *d_Mp);
+/* gpu_kernels.h:137-144 */
+int amsc_g2pd_gradients_and_height_update(
+    int np, int nrows, double hx, double hy, double dt, const double *d_x,
+    const double *d_y, const int *d_p2g, const double *d_vvxL,
+    const double *d_vvyL, double *d_vpx_dx, double *d_vpx_dy,
+    double *d_vpy_dx, double *d_vpy_dy, double *d_hp, double *d_Vp,
+    double *d_Ap, double *min_sc_out, double *min_hp_out);
+/* gpu_kernels.h:149-152 */
+int amsc_voellmy_friction(int np, double rho, double g_c, double tan_fa,
+                          double xi, double fric_on, const double *d_vpx,
+                          const double *d_vpy, const double *d_hp,
+                          double *d_Fb_x, double *d_Fb_y);
+/* gpu_kernels.h:157-167 */
+int amsc_bingham_stress_update(
+    int np, double rho, double g_c, double mu, double tau_Y, double cc,
+    double A_coeff, double C_coeff, const double *d_vpx, const double *d_vpy,
+    const double *d_hp, const double *d_H, const double *d_vpx_dx,
+    const double *d_vpx_dy, const double *d_vpy_dx, const double *d_vpy_dy,
+    double *d_F11, double *d_F12, double *d_F21, double *d_F22,
+    double *max_coeff_out, double *min_hp_out, double *max_nv_out,
+    double *max_F11_out, double *max_dev_out, double *max_pres_out,
+    double *max_Dxx_out);
+/* gpu_kernels.h:172-176 */
+int amsc_g2p_height(int np, int nrows, double hx, double hy,
+                    const double *d_x, const double *d_y, const int *d_p2g,
+                    const double *d_Z, const double *d_hp, double *d_Zp,
+                    double *d_hpZ);
+/* gpu_kernels.h:181-188 */
+int amsc_scatter_plotvars(int np, int nrows, double hx, double hy,
+                          const int *cstart, const int *cptcls,
+                          const int *ccidx, const int *coloff,
+                          const double *d_x, const double *d_y,
+                          const double *d_Mp, const double *d_vpx,
+                          const double *d_vpy, const double *d_apx,
+                          const double *d_apy, const double *d_hp,
+                          double *d_pv_rho, double *d_pv_vvx,
+                          double *d_pv_vvy, double *d_pv_avx,
+                          double *d_pv_avy, double *d_HV);
+/* gpu_kernels.h:193-196 */
+int amsc_normalize_plotvars(int nn, double *d_pv_rho, double *d_pv_vvx,
+                            double *d_pv_vvy, double *d_pv_avx,
+                            double *d_pv_avy);
+
+/* ------------------------------------------------------------------ 2. */
+/* Replaces quadgrid particles_t::init_particle_mesh() (called at
+ * optim_par.cpp:341 and gpu_offload.cpp:243,380,549, rebuilt on the host
+ * every step at gpu_offload.cpp:546-558).  Device arrays: p2g[np] (cell of
+ * each particle, floor(x/hx), floor(y/hy) clamped to the grid, cell id
+ * r + c*nrows), cstart[nrows*ncols+1], cptcls[np] (particle ids by cell,
+ * ascending id within a cell: a stable sort), ccidx[nrows*ncols] (cells of
+ * colour (r&1)+2*(c&1), colours 0..3, ascending within a colour);
+ * coloff[5] is host memory.  Any output pointer except p2g/cstart/cptcls may
+ * be NULL. */
+int amsc_bin_particles(int np, int nrows, int ncols, double hx, double hy,
+                       const double *d_x, const double *d_y, int *d_p2g,
+                       int *d_cstart, int *d_cptcls, int *d_ccidx,
+                       int *coloff);
+
+/* Solver parameters: the DATA.json scalars (main/mpm_data.h:14-56), names in
+ * lower case; nrows = Ney, ncols = Nex; phi in degrees; max_steps > 0 caps
+ * an adaptive run; ms_* as DATA's merge/split keys. */
+typedef struct amsc_params {
+  int nrows, ncols;
+  double hx, hy;
+  double g, rho, mu, tauy, xi, phi;
+  double bingham_on, friction_on, cfl, eq_level, dt_fixed, T;
+  int bc_flag, nsteps, max_steps;
+  int ms_on;
+  double ms_alpha, ms_beta, ms_split_hp_min, ms_shear_split, ms_hp_min,
+      ms_max_dv, ms_max_ops;
+  int ms_min_level, ms_max_level, ms_min_ppc, ms_interval;
+} amsc_params;
+
+/* decide_actions (merge_split_ops_cmes.h:116-253) on the device, for a state
+ * already binned by amsc_bin_particles (cstart/cptcls).  Writes
+ * action[np] (0 KEEP, 1 SPLIT, 2 MERGE_PRIMARY, 3 MERGE_SECONDARY) and
+ * partner[np] (device); counts (host, int[2]) = splits, merges.  Identical
+ * integer results to the serial reference for every ms_max_ops. */
+int amsc_ms_decide(const amsc_params *p, int np, double dt, const double *d_x,
+                   const double *d_y, const double *d_Ap, const double *d_hp,
+                   const double *d_vpx_dx, const double *d_vpx_dy,
+                   const double *d_vpy_dx, const double *d_vpy_dy,
+                   const double *d_vpx, const double *d_vpy,
+                   const int *d_level, const int *d_cstart,
+                   const int *d_cptcls, int *d_action, int *d_partner,
+                   int *counts);
+
+/* ------------------------------------------------------------------ 3. */
+typedef struct amsc_sim amsc_sim;
+
+/* Create a device-resident simulation from host arrays (DATA.json layout:
+ * particle arrays of length np, Z of length (nrows+1)*(ncols+1)).
+ * capacity >= np bounds the particle count merge/split may reach (0 = 2*np).
+ * Performs the step-0 initialisation of optim_par.cpp:235-264. */
+int amsc_sim_create(const amsc_params *p, int np, const double *x,
+                    const double *y, const double *Mp, const double *Ap,
+                    const double *vpx, const double *vpy,
+                    const double *mom_px, const double *mom_py,
+                    const double *hp, const double *Vp, const double *Z,
+                    int capacity, amsc_sim **out);
+/* Same, reading a DATA.json file (main/mpm_data.h schema). */
+int amsc_sim_create_from_json(const char *path, int max_steps, int capacity,
+                              amsc_sim **out);
+int amsc_sim_destroy(amsc_sim *s);
+/* Advance up to nsteps steps (fewer if the run ends: it == NSTEPS for a
+ * fixed dt, t >= T or it == max_steps for an adaptive one).  *done (may be
+ * NULL) receives the number of steps taken.  Synchronises only on
+ * merge/split steps and when the end condition needs reading. */
+int amsc_sim_step(amsc_sim *s, int nsteps, int *done);
+/* 1: every step also writes the output-only fields (Ap, Fb, velocity
+ * gradients, apx/apy, Zp, hpZ, dZxp/dZyp, Fpx/Fpy, mom) and the nodal
+ * arrays; 0 (default): only when a merge/split step needs them. */
+int amsc_sim_set_aux(amsc_sim *s, int on);
+/* Record HIP events around every kernel launch (per-kernel timing). */
+int amsc_sim_set_timing(amsc_sim *s, int on);
+/* Accumulated kernel time: names[i] (static strings), total ms and launch
+ * count for up to cap kernels; returns the number of kernels in *n. */
+int amsc_sim_kernel_times(amsc_sim *s, const char **names, double *ms,
+                          int *count, int cap, int *n);
+int amsc_sim_reset_kernel_times(amsc_sim *s);
+int amsc_sim_num_particles(amsc_sim *s, int *np);
+/* it (steps taken), t, dt of the last step; synchronises. */
+int amsc_sim_state(amsc_sim *s, int *it, double *t, double *dt);
+/* Copy a field to host memory (n entries at most); synchronises.
+ * Particle fields: x y Mp Ap vpx vpy mom_px mom_py hp Vp H F_11 F_12 F_21
+ * F_22 Fb_x Fb_y vpx_dx vpx_dy vpy_dx vpy_dy apx apy dZxp dZyp Zp hpZ Fpx Fpy
+ * Fric_px Fric_py vpxL vpyL; nodal: Mv mom_vx mom_vy F_ext_vx F_ext_vy
+ * F_int_vx F_int_vy Fric_x Fric_y FPxv FPyv Ftot_vx Ftot_vy avx avy vvx vvy
+ * vvxL vvyL Z. */
+int amsc_sim_get_field(amsc_sim *s, const char *name, double *out, int n);
+/* Integer fields: label level p2g cptcls cstart. */
+int amsc_sim_get_field_int(amsc_sim *s, const char *name, int *out, int n);
+/* Overwrite a particle field from host memory (n == num_particles). */
+int amsc_sim_set_field(amsc_sim *s, const char *name, const double *in,
+                       int n);
+/* err_mass, err_mom (optim_par.cpp:395-428, FP64 device sums) and t of the
+ * last step. */
+int amsc_sim_conservation(amsc_sim *s, double out[3]);
+/* splits, merges of the last merge/split call. */
+int amsc_sim_ms_counts(amsc_sim *s, int out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

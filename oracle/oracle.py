@@ -1,0 +1,301 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/mpm_oracle.c) and of the
+reference's own main/gpu_kernels.cpp built for the host (oracle/_ref).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, as the checker.  The product (amsc_amd/)
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpm_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_gpu_kernels.so")
+REF_SYMS = os.path.join(HERE, "_ref", "symbols.txt")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class OrParams(C.Structure):
+    """Mirror of or_params (mpm_oracle.h); field names follow DATA.json keys
+    in lower case (main/mpm_data.h:8-56)."""
+
+    _fields_ = [
+        ("nrows", C.c_int), ("ncols", C.c_int),
+        ("hx", C.c_double), ("hy", C.c_double),
+        ("g", C.c_double), ("rho", C.c_double), ("mu", C.c_double),
+        ("tauy", C.c_double), ("xi", C.c_double), ("phi", C.c_double),
+        ("bingham_on", C.c_double), ("friction_on", C.c_double),
+        ("cfl", C.c_double), ("eq_level", C.c_double),
+        ("dt_fixed", C.c_double), ("T", C.c_double),
+        ("bc_flag", C.c_int), ("nsteps", C.c_int), ("max_steps", C.c_int),
+        ("ms_on", C.c_int),
+        ("ms_alpha", C.c_double), ("ms_beta", C.c_double),
+        ("ms_split_hp_min", C.c_double), ("ms_shear_split", C.c_double),
+        ("ms_hp_min", C.c_double), ("ms_max_dv", C.c_double),
+        ("ms_max_ops", C.c_double),
+        ("ms_min_level", C.c_int), ("ms_max_level", C.c_int),
+        ("ms_min_ppc", C.c_int), ("ms_interval", C.c_int),
+    ]
+
+
+def params_struct(prm: dict, cls=OrParams):
+    """Fill a ctypes params structure from a flat dict of the same names."""
+    s = cls()
+    for name, typ in cls._fields_:
+        setattr(s, name, typ(prm[name]).value)
+    return s
+
+
+def ensure_built():
+    if not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "libmpm_oracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        ensure_built()
+        L = C.CDLL(LIB_PATH)
+        L.or_sim_create.restype = C.c_void_p
+        L.or_sim_create.argtypes = [C.POINTER(OrParams), C.c_int] + [_dp] * 10 + [_dp]
+        L.or_sim_destroy.argtypes = [C.c_void_p]
+        for f in ("or_sim_step", "or_sim_np", "or_sim_it"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_int
+        for f in ("or_sim_t", "or_sim_dt"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_double
+        L.or_sim_get.argtypes = [C.c_void_p, C.c_char_p, _dp]
+        L.or_sim_get.restype = C.c_int
+        L.or_sim_get_int.argtypes = [C.c_void_p, C.c_char_p, _ip]
+        L.or_sim_get_int.restype = C.c_int
+        L.or_sim_set.argtypes = [C.c_void_p, C.c_char_p, _dp]
+        L.or_sim_set.restype = C.c_int
+        L.or_sim_conservation.argtypes = [C.c_void_p, _dp]
+        L.or_sim_ms_counts.argtypes = [C.c_void_p, _ip]
+        L.or_bin_particles.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                       C.c_double, C.c_double, _ip, _ip, _ip,
+                                       _ip, _ip]
+        L.or_cell_of.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_double]
+        L.or_cell_of.restype = C.c_int
+        L.or_ms_decide.argtypes = [C.POINTER(OrParams), C.c_int, C.c_double] + \
+            [_dp] * 10 + [_ip, _ip, _ip, _ip]
+        L.or_stress_optim_par.argtypes = [C.c_int, C.POINTER(OrParams)] + [_dp] * 12
+        _lib = L
+    return _lib
+
+
+def stress_optim_par(prm: dict, vpx, vpy, hp, H, vpx_dx, vpx_dy, vpy_dx, vpy_dy):
+    """Oracle stress_tensor_t over arrays; returns F11, F12, F21, F22."""
+    n = len(vpx)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in
+            (vpx, vpy, hp, H, vpx_dx, vpx_dy, vpy_dx, vpy_dy)]
+    outs = [np.zeros(n) for _ in range(4)]
+    P = params_struct(prm)
+    lib().or_stress_optim_par(n, C.byref(P), *arrs, *outs)
+    return outs
+
+
+def bin_particles(x, y, nrows, ncols, hx, hy):
+    """Oracle binning: returns p2g, cstart, cptcls, ccidx, coloff."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    np_ = len(x)
+    nc = nrows * ncols
+    p2g = np.zeros(max(np_, 1), np.int32)
+    cptcls = np.zeros(max(np_, 1), np.int32)
+    cstart = np.zeros(nc + 1, np.int32)
+    ccidx = np.zeros(max(nc, 1), np.int32)
+    coloff = np.zeros(5, np.int32)
+    lib().or_bin_particles(np_, x, y, nrows, ncols, hx, hy, p2g, cstart,
+                           cptcls, ccidx, coloff)
+    return p2g[:np_], cstart, cptcls[:np_], ccidx[:nc], coloff
+
+
+def ms_decide(prm: dict, dt, x, y, Ap, hp, vpx_dx, vpx_dy, vpy_dx, vpy_dy,
+              vpx, vpy, level):
+    n = len(x)
+    act = np.zeros(max(n, 1), np.int32)
+    partner = np.zeros(max(n, 1), np.int32)
+    counts = np.zeros(2, np.int32)
+    P = params_struct(prm)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in
+            (x, y, Ap, hp, vpx_dx, vpx_dy, vpy_dx, vpy_dy, vpx, vpy)]
+    lib().or_ms_decide(C.byref(P), n, dt, *arrs,
+                       np.ascontiguousarray(level, np.int32), act, partner,
+                       counts)
+    return act[:n], partner[:n], counts
+
+
+class OracleSim:
+    """optim_par time loop (main/optim_par.cpp:281-668) + merge/split at
+    gpu_offload's position (main/gpu_offload.cpp:321-401)."""
+
+    def __init__(self, prm: dict, particles: dict, Z):
+        L = lib()
+        self._P = params_struct(prm)
+        self.prm = prm
+        self.nn = (prm["nrows"] + 1) * (prm["ncols"] + 1)
+        a = [np.ascontiguousarray(particles[k], np.float64) for k in
+             ("x", "y", "Mp", "Ap", "vpx", "vpy", "mom_px", "mom_py", "hp", "Vp")]
+        self._h = L.or_sim_create(C.byref(self._P), len(a[0]), *a,
+                                  np.ascontiguousarray(Z, np.float64))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_sim_destroy(self._h)
+            self._h = None
+
+    def step(self) -> bool:
+        return bool(lib().or_sim_step(self._h))
+
+    def run(self, nsteps: int) -> int:
+        done = 0
+        for _ in range(nsteps):
+            if not self.step():
+                break
+            done += 1
+        return done
+
+    @property
+    def np(self):
+        return lib().or_sim_np(self._h)
+
+    @property
+    def it(self):
+        return lib().or_sim_it(self._h)
+
+    @property
+    def t(self):
+        return lib().or_sim_t(self._h)
+
+    @property
+    def dt(self):
+        return lib().or_sim_dt(self._h)
+
+    def get(self, name):
+        out = np.zeros(max(self.np, self.nn, 1), np.float64)
+        n = lib().or_sim_get(self._h, name.encode(), out)
+        if n < 0:
+            raise KeyError(name)
+        return out[:n].copy()
+
+    def get_int(self, name):
+        nc = self.prm["nrows"] * self.prm["ncols"]
+        out = np.zeros(max(self.np, nc + 1, 1), np.int32)
+        n = lib().or_sim_get_int(self._h, name.encode(), out)
+        if n < 0:
+            raise KeyError(name)
+        return out[:n].copy()
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, np.float64)
+        assert len(a) == self.np
+        if lib().or_sim_set(self._h, name.encode(), a) < 0:
+            raise KeyError(name)
+
+    def conservation(self):
+        out = np.zeros(3, np.float64)
+        lib().or_sim_conservation(self._h, out)
+        return out
+
+    def ms_counts(self):
+        out = np.zeros(2, np.int32)
+        lib().or_sim_ms_counts(self._h, out)
+        return out
+
+
+# ---------------------------------------------------------------------------
+# per-kernel restatements (or_*) and the reference's gpu_kernels (host build)
+# ---------------------------------------------------------------------------
+# Argument kinds per entry point, in gpu_kernels.h order:
+#   i int, d double, b bool, I int* (device), H int* read on the host,
+#   P const double*, O double* (in/out).
+SIGNATURES = {
+    "reset_nodal_arrays": "i" + "O" * 25,
+    "scatter_mass_momentum": "iiddIIIH" + "PP" + "PPP" + "OOO",
+    "gradZ_to_particles": "iidd" + "PPI" + "P" + "OO",
+    "gravity_force": "id" + "PPPPP" + "OO",
+    "friction_particle_force": "i" + "PPP" + "OO",
+    "scatter_gravity_force": "iiddIIIH" + "PP" + "PP" + "OO",
+    "scatter_friction_force": "iiddIIIH" + "PP" + "PP" + "OO",
+    "compute_F_ext": "i" + "PPPP" + "OO",
+    "scatter_internal_forces": "iiddIIIH" + "PPP" + "PPPP" + "OO",
+    "nodal_force_and_momentum": "id" + "PPPP" + "OOOO",
+    "nodal_velocity_and_bc": "iiidb" + "PPP" + "OOOOOOOO",
+    "g2p_velocity_and_advect": "iiiddd" + "I" + "PPPP" + "OOOOOOOOOO" + "P",
+    "g2pd_gradients_and_height_update": "iiddd" + "PPI" + "PP" + "OOOOOOO" + "OO",
+    "voellmy_friction": "iddddd" + "PPP" + "OO",
+    "bingham_stress_update": "iddddddd" + "PPPPPPPP" + "OOOO" + "OOOOOOO",
+    "g2p_height": "iidd" + "PPI" + "PP" + "OO",
+    "scatter_plotvars": "iiddIIIH" + "PP" + "PPPPPP" + "OOOOOO",
+    "normalize_plotvars": "i" + "OOOOO",
+}
+
+_KIND = {"i": C.c_int, "d": C.c_double, "b": C.c_bool, "I": _ip, "H": _ip,
+         "P": _dp, "O": _dp}
+
+
+def _bind(fn, sig):
+    fn.argtypes = [_KIND[k] for k in sig]
+    fn.restype = None
+    return fn
+
+
+def oracle_kernel(name):
+    """or_<name> restatement.  g2pd_gradients_and_height_update and
+    bingham_stress_update take no diagnostic *_out pointers in the oracle
+    (the reference never writes them, gpu_kernels.cpp:450-453,539-545); the
+    wrapper accepts and drops them so both share one call signature."""
+    sig = SIGNATURES[name].replace("b", "i")  # or_* take int for bool
+    drop = 0
+    if name == "g2pd_gradients_and_height_update":
+        drop = 2
+    elif name == "bingham_stress_update":
+        drop = 7
+    fn = _bind(getattr(lib(), "or_" + name), sig[:len(sig) - drop] if drop else sig)
+    if not drop:
+        return fn
+
+    def call(*args):
+        return fn(*args[:len(args) - drop])
+    return call
+
+
+_ref = None
+_ref_syms = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH) and os.path.exists(REF_SYMS)
+
+
+def ref_kernel(name):
+    """The reference's own gpu_kernels::<name>, compiled for the host."""
+    global _ref, _ref_syms
+    if _ref is None:
+        _ref = C.CDLL(REF_PATH)
+        with open(REF_SYMS) as f:
+            syms = [s.strip() for s in f if s.strip()]
+        _ref_syms = {}
+        for s in syms:
+            # _ZN11gpu_kernels<len><name>E...
+            body = s[len("_ZN11gpu_kernels"):]
+            i = 0
+            while body[i].isdigit():
+                i += 1
+            ln = int(body[:i])
+            _ref_syms[body[i:i + ln]] = s
+    return _bind(getattr(_ref, _ref_syms[name]), SIGNATURES[name])
