@@ -273,8 +273,12 @@ __global__ void __launch_bounds__(kP2GThreads)
         const int jp = s_cs[s][0] + (q - s_sego[s]);
         const int ip = a.cptcls[jp];
         const double x = a.p.f[PF_X][ip], y = a.p.f[PF_Y][ip];
-        const int cell = cell_of(x, y, nrows, ncols, hx, hy);
-        const int pr = cell % nrows, pc = cell / nrows;
+        // The CSR segment fixes the cell column; the row is the binning's
+        // floor(y/hy), clamped to the staged rows so LDS indices stay in
+        // bounds whatever the input.
+        const int pc = ccl + s;
+        int pr = cell_of(x, y, nrows, ncols, hx, hy) % nrows;
+        pr = pr < rl ? rl : (pr > rh ? rh : pr);
         const Local Lc = local_coords(x, y, pc, pr, hx, hy);
         // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
         const int zb = (pc - (c0 - 1)) * (kTile + 2) + (pr - (r0 - 1));
@@ -746,6 +750,7 @@ struct Sim {
   int ms_counts[2] = {0, 0};
   // host mirrors
   int it_host = 0;
+  long long particle_steps = 0;  // sum of np over launched active steps
   bool finished = false;
   bool aux_on = false;
   bool diag = true;
@@ -1006,6 +1011,7 @@ int launch_step(Sim &s) {
   s.aux_valid = aux;
   s.nodal_valid = aux;
   s.it_host++;
+  s.particle_steps += s.np;
   if (s.p.dt_fixed > 0.0 && s.it_host >= s.p.nsteps) s.finished = true;
   if (s.p.dt_fixed <= 0.0 && s.p.max_steps > 0 && s.it_host >= s.p.max_steps)
     s.finished = true;
@@ -1299,6 +1305,12 @@ int amsc_sim_conservation(amsc_sim *s, double out[3]) {
   out[0] = s->ctrl_host->cons[0];
   out[1] = s->ctrl_host->cons[1];
   out[2] = s->ctrl_host->t_start;
+  return AMSC_OK;
+}
+
+int amsc_sim_particle_steps(amsc_sim *s, long long *out) {
+  if (!s || !out) return fail(AMSC_ERR_INVALID, "bad arguments");
+  *out = s->particle_steps;
   return AMSC_OK;
 }
 

@@ -127,6 +127,7 @@ def lib():
     L.amsc_sim_set_field.argtypes = [_vp, C.c_char_p, _dp, C.c_int]
     L.amsc_sim_conservation.argtypes = [_vp, _dp]
     L.amsc_sim_ms_counts.argtypes = [_vp, _ip]
+    L.amsc_sim_particle_steps.argtypes = [_vp, C.POINTER(C.c_longlong)]
     _lib = L
     return L
 

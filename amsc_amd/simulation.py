@@ -124,6 +124,22 @@ class Simulation:
         native.check(native.lib().amsc_sim_conservation(self._h, out))
         return out
 
+    @property
+    def particle_steps(self) -> int:
+        out = C.c_longlong()
+        native.check(native.lib().amsc_sim_particle_steps(self._h, C.byref(out)))
+        return out.value
+
+    def active_nodes(self) -> int:
+        """Nodes that are corners of occupied cells (current binning)."""
+        cs = self.get_int("cstart")
+        occ = np.nonzero(np.diff(cs) > 0)[0]
+        nrows = self.prm["nrows"]
+        r, c = occ % nrows, occ // nrows
+        nodes = np.concatenate([r + c * (nrows + 1), r + 1 + c * (nrows + 1),
+                                r + (c + 1) * (nrows + 1), r + 1 + (c + 1) * (nrows + 1)])
+        return int(np.unique(nodes).size)
+
     def ms_counts(self) -> np.ndarray:
         out = np.zeros(2, np.int32)
         native.check(native.lib().amsc_sim_ms_counts(self._h, out))

@@ -271,6 +271,10 @@ int amsc_sim_set_field(amsc_sim *s, const char *name, const double *in,
 /* err_mass, err_mom (optim_par.cpp:395-428, FP64 device sums) and t of the
  * last step. */
 int amsc_sim_conservation(amsc_sim *s, double out[3]);
+/* Particle-steps launched so far: the sum over steps of the particle count
+ * at that step (steps issued after the end of a run are not counted once
+ * the end has been observed). */
+int amsc_sim_particle_steps(amsc_sim *s, long long *out);
 /* splits, merges of the last merge/split call. */
 int amsc_sim_ms_counts(amsc_sim *s, int out[2]);
 

@@ -1,0 +1,67 @@
+"""Device binning (amsc_bin_particles) against the oracle's counting sort:
+identical integers (cell of each particle, cell starts, particle order, the
+colouring), including empty grids, crowded cells and cell-edge positions."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def device_bin(x, y, nrows, ncols, hx, hy):
+    import torch
+    from amsc_amd import native
+    n = len(x)
+    tx = torch.from_numpy(np.ascontiguousarray(x, np.float64)).cuda()
+    ty = torch.from_numpy(np.ascontiguousarray(y, np.float64)).cuda()
+    p2g = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    cptcls = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    cstart = torch.zeros(nrows * ncols + 1, dtype=torch.int32, device="cuda")
+    ccidx = torch.zeros(max(nrows * ncols, 1), dtype=torch.int32, device="cuda")
+    print(f"[bin] n={n} grid={nrows}x{ncols}", flush=True)
+    coloff = native.bin_particles(n, nrows, ncols, hx, hy, tx.data_ptr(),
+                                  ty.data_ptr(), p2g.data_ptr(), cstart.data_ptr(),
+                                  cptcls.data_ptr(), ccidx.data_ptr())
+    torch.cuda.synchronize()
+    return (p2g.cpu().numpy()[:n], cstart.cpu().numpy(), cptcls.cpu().numpy()[:n],
+            ccidx.cpu().numpy()[:nrows * ncols], coloff)
+
+
+def check(x, y, nrows, ncols, hx, hy):
+    got = device_bin(x, y, nrows, ncols, hx, hy)
+    exp = oracle.bin_particles(x, y, nrows, ncols, hx, hy)
+    for name, a, b in zip(("p2g", "cstart", "cptcls", "ccidx", "coloff"), got, exp):
+        assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("n,nrows,ncols", [(1000, 17, 23), (200000, 300, 257),
+                                           (1_000_000, 1000, 1000)])
+def test_random(n, nrows, ncols):
+    rng = np.random.default_rng(n)
+    hx, hy = 0.5, 0.75
+    x = rng.uniform(1e-10, ncols * hx - 1e-10, n)
+    y = rng.uniform(1e-10, nrows * hy - 1e-10, n)
+    x[: n // 20] = np.floor(x[: n // 20] / hx) * hx  # exactly on cell edges
+    check(x, y, nrows, ncols, hx, hy)
+
+
+def test_crowded_cells_and_sorted_input():
+    rng = np.random.default_rng(1)
+    nrows, ncols, hx, hy = 9, 7, 1.0, 1.0
+    # 300 particles in one cell (global in-cell sort path), 40 in another
+    x = np.concatenate([rng.uniform(2.0, 3.0, 300), rng.uniform(5.0, 6.0, 40),
+                        rng.uniform(0, 7, 300)])
+    y = np.concatenate([rng.uniform(4.0, 5.0, 300), rng.uniform(1.0, 2.0, 40),
+                        rng.uniform(0, 9, 300)])
+    perm = rng.permutation(len(x))
+    check(x[perm], y[perm], nrows, ncols, hx, hy)
+    xs, ys = np.sort(x), np.sort(y)
+    check(xs, ys, nrows, ncols, hx, hy)
+
+
+def test_empty_and_clamped():
+    check(np.zeros(0), np.zeros(0), 4, 5, 1.0, 1.0)
+    # outside the domain: clamped to the border cells
+    check(np.array([-1.0, 100.0, 2.5, 5.0]), np.array([2.0, -3.0, 50.0, 4.0]),
+          4, 5, 1.0, 1.0)

@@ -1,0 +1,75 @@
+"""Device merge/split decisions (amsc_ms_decide) against the oracle's
+restatement of decide_actions (merge_split_ops_cmes.h:116-253): identical
+action and partner arrays, with and without a binding ms_max_ops cap."""
+import numpy as np
+import pytest
+
+from amsc_amd import scenarios
+from amsc_amd.data import params_from_data
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def snapshot(seed, n=6000, nrows=30, ncols=40, h=1.0):
+    rng = np.random.default_rng(seed)
+    # clusters so that cells hold many candidates
+    cx = rng.uniform(0, ncols * h, 60)
+    cy = rng.uniform(0, nrows * h, 60)
+    k = rng.integers(0, 60, n)
+    x = np.clip(cx[k] + rng.normal(0, 0.6, n), 1e-10, ncols * h - 1e-10)
+    y = np.clip(cy[k] + rng.normal(0, 0.6, n), 1e-10, nrows * h - 1e-10)
+    s = dict(x=x, y=y,
+             Ap=rng.choice([0.01, 0.05, 0.2, 1.2], n) * rng.uniform(0.8, 1.2, n),
+             hp=rng.uniform(0.0, 2.0, n),
+             vpx_dx=rng.normal(0, 0.5, n), vpx_dy=rng.normal(0, 0.5, n),
+             vpy_dx=rng.normal(0, 0.5, n), vpy_dy=rng.normal(0, 0.5, n),
+             vpx=rng.normal(0, 0.02, n), vpy=rng.normal(0, 0.02, n),
+             level=rng.integers(-3, 4, n).astype(np.int32))
+    d = scenarios.dambreak_flat(1)
+    prm = params_from_data(d)
+    prm.update(nrows=nrows, ncols=ncols, hx=h, hy=h, ms_alpha=0.9, ms_beta=0.3,
+               ms_shear_split=0.05, ms_split_hp_min=0.2, ms_hp_min=0.05,
+               ms_max_dv=0.03, ms_min_ppc=2)
+    return prm, s
+
+
+def device_decide(prm, s, dt):
+    import torch
+    from amsc_amd import native
+    n = len(s["x"])
+    keep = []
+
+    def dev(a, dt_=torch.float64):
+        t = torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt_)
+        keep.append(t)
+        return t.data_ptr()
+    _, cstart, cptcls, _, _ = oracle.bin_particles(s["x"], s["y"], prm["nrows"],
+                                                   prm["ncols"], prm["hx"], prm["hy"])
+    ptrs = [dev(s[k]) for k in ("x", "y", "Ap", "hp", "vpx_dx", "vpx_dy",
+                                "vpy_dx", "vpy_dy", "vpx", "vpy")]
+    act = torch.zeros(n, dtype=torch.int32, device="cuda")
+    part = torch.zeros(n, dtype=torch.int32, device="cuda")
+    counts = native.ms_decide(prm, n, dt, ptrs, dev(s["level"], torch.int32),
+                              dev(cstart, torch.int32), dev(cptcls, torch.int32),
+                              act.data_ptr(), part.data_ptr())
+    torch.cuda.synchronize()
+    return act.cpu().numpy(), part.cpu().numpy(), counts
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("max_ops", [1e9, 25, 0])
+def test_decisions_identical(seed, max_ops):
+    prm, s = snapshot(seed)
+    prm["ms_max_ops"] = max_ops
+    dt = 0.05
+    print(f"[ms] seed={seed} max_ops={max_ops}", flush=True)
+    a1, p1, c1 = device_decide(prm, s, dt)
+    a2, p2, c2 = oracle.ms_decide(prm, dt, s["x"], s["y"], s["Ap"], s["hp"],
+                                  s["vpx_dx"], s["vpx_dy"], s["vpy_dx"],
+                                  s["vpy_dy"], s["vpx"], s["vpy"], s["level"])
+    assert np.array_equal(a1, a2)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(c1, c2)
+    if max_ops == 1e9:
+        assert (a1 == 1).sum() > 50 and (a1 == 2).sum() > 50  # non-trivial

@@ -1,0 +1,219 @@
+"""The device-resident FP64 time loop (amsc_sim_*) against the oracle's
+optim_par step on the same inputs.
+
+Every comparison of state is bit-for-bit: positions, velocities, depths,
+stresses, friction, gradients, nodal velocities, dt and t, particle counts,
+labels and levels after merge/split.  Conservation sums are computed on the
+device in a different (tree) order and are checked against the reference's
+1e-10 threshold (data/conservation_test/scripts/conservation_check.py:36).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from amsc_amd import scenarios
+from amsc_amd.data import load_data_json, params_from_data, particles_from_data
+from oracle import oracle
+from test_oracle_step import small_slope_case
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+PFIELDS = ("x", "y", "vpx", "vpy", "hp", "Vp", "Mp", "H", "F_11", "F_12",
+           "F_21", "F_22", "Ap", "Fb_x", "Fb_y", "vpx_dx", "vpx_dy", "vpy_dx",
+           "vpy_dy", "apx", "apy", "Zp", "hpZ", "dZxp", "dZyp", "Fpx", "Fpy",
+           "mom_px", "mom_py")
+NFIELDS = ("vvxL", "vvyL", "avx", "avy", "Mv", "mom_vx", "mom_vy", "F_ext_vx",
+           "F_ext_vy", "F_int_vx", "F_int_vy", "Fric_x", "Fric_y", "FPxv",
+           "FPyv", "Ftot_vx", "Ftot_vy", "vvx", "vvy")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.int64)
+
+
+def pair(d, capacity=0, **over):
+    from amsc_amd.simulation import Simulation
+    prm = params_from_data(d)
+    prm.update(over)
+    part = particles_from_data(d)
+    o = oracle.OracleSim(prm, part, d["Z"])
+    g = Simulation(prm, part, d["Z"], capacity=capacity)
+    g.set_aux(True)
+    return o, g, prm
+
+
+def assert_same(o, g, where, nodal=True):
+    assert g.num_particles == o.np, where
+    it, t, dt = g.state()
+    assert (it, t, dt) == (o.it, o.t, o.dt), where
+    for k in PFIELDS:
+        a, b = g.get(k), o.get(k)
+        assert np.array_equal(bits(a), bits(b)), f"{where}: {k} max|d|={np.abs(a - b).max()}"
+    for k in ("label", "level"):
+        assert np.array_equal(g.get_int(k), o.get_int(k)), f"{where}: {k}"
+    if nodal:
+        for k in NFIELDS:
+            a, b = g.get(k), o.get(k)
+            assert np.array_equal(bits(a), bits(b)), f"{where}: {k} max|d|={np.abs(a - b).max()}"
+
+
+def advance(o, g, n, check_cons=True):
+    worst = 0.0
+    for _ in range(n):
+        ok = o.step()
+        done = g.step(1)
+        assert done == int(ok)
+        if not ok:
+            break
+        if check_cons:
+            em, eo, t0 = g.conservation()
+            worst = max(worst, em, eo)
+    assert worst < 1e-10
+    return worst
+
+
+def test_dambreak_bitwise():
+    print("[sim] dambreak", flush=True)
+    o, g, _ = pair(scenarios.dambreak_flat(60))
+    assert_same(o, g, "init", nodal=False)
+    advance(o, g, 1)
+    assert_same(o, g, "step 1")
+    advance(o, g, 59)
+    assert_same(o, g, "step 60")
+    assert g.step(5) == 0  # fixed-dt run ends at NSTEPS
+
+
+def test_reference_fixture_bitwise():
+    """data/performance/DATA_bench_1000.json, 100 steps (merge/split on)."""
+    print("[sim] DATA_bench_1000", flush=True)
+    d = load_data_json(os.path.join(GOLDEN, "DATA_bench_1000.json"))
+    o, g, _ = pair(d)
+    advance(o, g, 37)
+    assert_same(o, g, "step 37")
+    advance(o, g, 63)
+    assert_same(o, g, "step 100")
+
+
+def test_json_loader_matches_python_loader():
+    from amsc_amd.simulation import Simulation
+    from amsc_amd import native
+    import ctypes as C
+    path = os.path.join(GOLDEN, "DATA_bench_1000.json")
+    d = load_data_json(path)
+    h = C.c_void_p()
+    native.check(native.lib().amsc_sim_create_from_json(path.encode(), 0, 0, C.byref(h)))
+    a = Simulation.__new__(Simulation)
+    a._h, a.prm, a.nn, a.nc = h, params_from_data(d), (d["Nex"] + 1) * (d["Ney"] + 1), d["Nex"] * d["Ney"]
+    b = Simulation.from_data(d)
+    for s in (a, b):
+        s.set_aux(True)
+        s.step(20)
+    for k in ("x", "y", "hp", "vpx", "F_11", "Ap"):
+        assert np.array_equal(bits(a.get(k)), bits(b.get(k))), k
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("eq", [0.0, 2.0])
+def test_adaptive_dt_slope_bitwise(eq):
+    print(f"[sim] adaptive eq={eq}", flush=True)
+    d = small_slope_case(eq=eq)
+    d["DT_FIXED"] = 0.0
+    d["CFL"] = 0.4
+    d["T"] = 0.35
+    o, g, _ = pair(d)
+    advance(o, g, 12)
+    assert_same(o, g, "step 12")
+    # run to the end time
+    while o.step():
+        pass
+    g.run()
+    assert_same(o, g, "end")
+    assert o.t >= d["T"]
+
+
+def test_merge_split_bitwise():
+    """Thresholds chosen so both splits and merges happen every 4 steps."""
+    print("[sim] merge/split", flush=True)
+    d = small_slope_case()
+    d.update(MERGE_SPLIT_ON=1, ms_call_interval=4, ms_alpha=0.3, ms_beta=0.52,
+             ms_max_dv=2.0, ms_hp_min=0.0, NSTEPS=40)
+    o, g, _ = pair(d, capacity=8 * len(d["x"]))
+    splits = merges = 0
+    for k in range(40):
+        advance(o, g, 1)
+        if (k + 1) % 4 == 0:
+            assert np.array_equal(g.ms_counts(), o.ms_counts()), f"step {k + 1}"
+            splits += int(o.ms_counts()[0])
+            merges += int(o.ms_counts()[1])
+            assert_same(o, g, f"step {k + 1}")
+    assert splits > 50 and merges > 20, (splits, merges)
+
+
+def test_first_step_uses_data_momentum():
+    """DATA momentum that is not vpx*Mp feeds the first P2G
+    (optim_par.cpp:151-152,383); afterwards mom = vpx*Mp."""
+    d = small_slope_case()
+    rng = np.random.default_rng(4)
+    d["vpx"] = rng.normal(0, 0.3, len(d["x"]))
+    d["vpy"] = rng.normal(0, 0.3, len(d["x"]))
+    d["mom_px"] = d["vpx"] * d["Mp"] * 1.01
+    d["mom_py"] = d["vpy"] * d["Mp"] * 0.99
+    o, g, _ = pair(d)
+    advance(o, g, 3, check_cons=False)
+    assert_same(o, g, "step 3")
+
+
+def test_crowded_tiles_bitwise():
+    """~200 particles per cell in a 10x10-cell patch: P2G tiles stage their
+    particles through LDS in many chunks."""
+    print("[sim] crowded", flush=True)
+    rng = np.random.default_rng(8)
+    d = small_slope_case(nrows=40, ncols=40, hx=0.25, hy=0.25)
+    n = 20000
+    x = rng.uniform(2.5, 5.0, n)
+    y = rng.uniform(2.5, 5.0, n)
+    hp = rng.uniform(0.2, 1.0, n)
+    Ap = np.full(n, 0.25 * 0.25 / 200)
+    d.update(x=x, y=y, hp=hp, Ap=Ap, Vp=hp * Ap, Mp=2000.0 * hp * Ap,
+             vpx=np.zeros(n), vpy=np.zeros(n), mom_px=np.zeros(n),
+             mom_py=np.zeros(n), DT_FIXED=5e-4, NSTEPS=6)
+    o, g, _ = pair(d)
+    advance(o, g, 6)
+    assert_same(o, g, "step 6")
+
+
+def test_large_landslide_properties():
+    """The ~2M-particle slope landslide: one step bit-for-bit against the
+    oracle, then 40 adaptive steps (merge/split every 10) with mass transfer
+    to round-off every step and total particle mass unchanged by merges."""
+    from amsc_amd.simulation import Simulation
+    print("[sim] large landslide", flush=True)
+    d = scenarios.slope_landslide(max_steps=41)
+    prm = params_from_data(d)
+    part = particles_from_data(d)
+    g = Simulation(prm, part, d["Z"], capacity=3 * len(d["x"]))
+    g.set_aux(True)
+    o = oracle.OracleSim(prm, part, d["Z"])
+    o.step()
+    g.step(1)
+    for k in ("x", "y", "vpx", "vpy", "hp", "F_11", "F_12", "F_22", "Ap", "Fb_x"):
+        assert np.array_equal(bits(g.get(k)), bits(o.get(k))), k
+    del o
+    g.set_aux(False)
+    m0 = d["Mp"].sum()
+    worst = 0.0
+    for _ in range(40):
+        assert g.step(1) == 1
+        em, eo, _ = g.conservation()
+        worst = max(worst, em, eo)
+    assert worst < 1e-10
+    g.set_aux(True)
+    g.step(1)
+    Mp = g.get("Mp")
+    assert abs(Mp.sum() - m0) <= 1e-12 * m0
+    assert np.isfinite(g.get("x")).all() and np.isfinite(g.get("hp")).all()
+    g.close()
