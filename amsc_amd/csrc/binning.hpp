@@ -44,23 +44,30 @@ int launch_colouring(int nrows, int ncols, int *ccidx, int *coloff,
                      hipStream_t st);
 
 // Wave-aggregated atomic increment of count[key]: lanes of a wavefront with
-// equal keys share one atomic; the returned slot is the lane's rank among
-// them (ascending lane = ascending particle id) plus the base.
+// equal keys form a group whose lowest lane (the leader) adds the group size
+// with one atomic; a lane's slot is the returned base plus its rank in the
+// group (ascending lane = ascending particle id).  Groups are found with
+// ballots first, then all leaders issue their atomics together, so the
+// wavefront waits for one atomic round trip, not one per distinct key.
 __device__ __forceinline__ int wave_slot(int *count, int key, bool valid) {
   unsigned long long todo = __ballot(valid);
-  int slot = 0;
-  const unsigned lane = __lane_id();
+  const int lane = (int)__lane_id();
+  int my_leader = lane, my_rank = 0, my_count = 0;
   while (todo) {
     const int leader = __ffsll((long long)todo) - 1;
     const int lkey = __shfl(key, leader);
     const unsigned long long peers = __ballot(valid && key == lkey) & todo;
-    int base = 0;
-    if ((int)lane == leader) base = atomicAdd(&count[lkey], __popcll(peers));
-    base = __shfl(base, leader);
-    if ((peers >> lane) & 1ull) slot = base + __popcll(peers & lanemask_lt());
+    if ((peers >> lane) & 1ull) {
+      my_leader = leader;
+      my_rank = __popcll(peers & lanemask_lt());
+    }
+    if (lane == leader) my_count = __popcll(peers);
     todo &= ~peers;
   }
-  return slot;
+  int base = 0;
+  if (valid && lane == my_leader) base = atomicAdd(&count[key], my_count);
+  base = __shfl(base, my_leader);
+  return base + my_rank;
 }
 
 }  // namespace amsc

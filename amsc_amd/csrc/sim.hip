@@ -87,7 +87,10 @@ struct Ctrl {
   double dt;       // dt of the last step taken
   double t_start;  // time at the start of the last step taken
   double cons[2];  // err_mass, err_mom of the last step
-  unsigned long long cfl[2][3];  // max |vpx|, |vpy|, hp bits (by step parity)
+  // max |vpx|, |vpy|, hp of the particles as bit patterns of non-negative
+  // doubles, by step parity, spread over kCflSlots addresses so that the
+  // per-block atomics of the particle kernel do not all hit one line
+  unsigned long long cfl[2][64][3];
   int it;
   int active;
 };
@@ -102,44 +105,67 @@ struct KParams {
 // ------------------------------------------------------------- kernels
 namespace {
 
-constexpr int kTile = 16;              // nodes per tile side
-constexpr int kSeg = kTile + 1;        // cell columns (incl. halo)
-constexpr int kChunk = 256;            // particles staged per round
-constexpr int kP2GThreads = kTile * kTile;
+constexpr int kTC = 16;                // node columns per P2G tile
+constexpr int kTR = 32;                // node rows per P2G tile
+constexpr int kSeg = kTC + 1;          // cell columns (incl. halo)
+constexpr int kP2GThreads = kTC * kTR; // one node per thread
+constexpr int kChunk = kP2GThreads;    // particles staged per round
 constexpr int kNS = 13;                // staged values per particle
 
-__global__ void prologue_kernel(KParams P, Ctrl *ctrl) {
+// Which optional fields a step writes: none; what the next merge/split
+// reads (Ap, Fb, velocity gradients); every output field and the nodal set.
+enum : int { AUX_NONE = 0, AUX_MS = 1, AUX_FULL = 2 };
+
+constexpr int kCflSlots = 64;
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(v, d);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// One wavefront: end test, dt, t, it for the step about to run.
+__global__ void __launch_bounds__(64) prologue_kernel(KParams P, Ctrl *ctrl) {
+  const int lane = threadIdx.x;
   if (!ctrl->active) return;
   const bool fixed = P.dt_fixed > 0.0;
   const int it = ctrl->it;
   // loop condition, optim_par.cpp:281; optional step cap for adaptive runs
-  if (fixed ? !(it < P.nsteps) : !(ctrl->t < P.T)) {
-    ctrl->active = 0;
+  const bool ended = (fixed ? !(it < P.nsteps) : !(ctrl->t < P.T)) ||
+                     (!fixed && P.max_steps > 0 && it >= P.max_steps);
+  if (ended) {
+    if (lane == 0) ctrl->active = 0;
     return;
   }
-  if (!fixed && P.max_steps > 0 && it >= P.max_steps) {
-    ctrl->active = 0;
-    return;
-  }
-  double dt = ctrl->dt;
   const int s = it + 1;  // this step
+  const unsigned long long m0 = wave_max_u64(ctrl->cfl[(s - 1) & 1][lane][0]);
+  const unsigned long long m1 = wave_max_u64(ctrl->cfl[(s - 1) & 1][lane][1]);
+  const unsigned long long m2 = wave_max_u64(ctrl->cfl[(s - 1) & 1][lane][2]);
+  double dt = ctrl->dt;
   if (fixed) {
     dt = P.dt_fixed;
   } else if (it > 0) {
     // optim_par.cpp:289-304: max |v| and max hp over the particles
-    const double mvx = __longlong_as_double((long long)ctrl->cfl[(s - 1) & 1][0]);
-    const double mvy = __longlong_as_double((long long)ctrl->cfl[(s - 1) & 1][1]);
-    const double hmax = __longlong_as_double((long long)ctrl->cfl[(s - 1) & 1][2]);
+    const double mvx = __longlong_as_double((long long)m0);
+    const double mvy = __longlong_as_double((long long)m1);
+    const double hmax = __longlong_as_double((long long)m2);
     const double a = sqrt(P.g * hmax) + mvx, b = sqrt(P.g * hmax) + mvy;
     const double cel = fabs(a > b ? a : b);
     const double minh = P.hx < P.hy ? P.hx : P.hy;
     dt = P.cfl * minh / (1e-2 + cel);
   }
-  ctrl->dt = dt;
-  ctrl->t_start = ctrl->t;
-  ctrl->t = ctrl->t + dt;
-  ctrl->it = s;
-  ctrl->cfl[s & 1][0] = ctrl->cfl[s & 1][1] = ctrl->cfl[s & 1][2] = 0ull;
+  ctrl->cfl[s & 1][lane][0] = 0ull;
+  ctrl->cfl[s & 1][lane][1] = 0ull;
+  ctrl->cfl[s & 1][lane][2] = 0ull;
+  if (lane == 0) {
+    ctrl->dt = dt;
+    ctrl->t_start = ctrl->t;
+    ctrl->t = ctrl->t + dt;
+    ctrl->it = s;
+  }
 }
 
 // Step-0 initialisation, optim_par.cpp:235-264 (cells from the initial
@@ -186,7 +212,7 @@ struct P2GArgs {
   const Ctrl *ctrl;
   NPtr n;
   double *partials;  // [nblocks][6] or null
-  int aux;           // write avx/avy, nodal dump and P2G particle outputs
+  int aux;           // AUX_FULL: avx/avy, nodal dump, P2G particle outputs
   int mom_stored;    // momentum from the mom arrays (first step only)
 };
 
@@ -194,6 +220,37 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
   return v;
+}
+
+// One particle's P2G inputs, gathered through cptcls.
+struct P2GRaw {
+  double x, y, Mp, m1, m2, Vp, hp, H, frx, fry, f11, f12, f22;
+  int ip, s;
+};
+
+__device__ __forceinline__ void p2g_fetch(const P2GArgs &a, int q, int nseg,
+                                          const int (*s_cs)[kTR + 2],
+                                          const int *s_sego, P2GRaw &R) {
+  int s = 0;
+  while (s + 1 < nseg && s_sego[s + 1] <= q) ++s;
+  const int ip = a.cptcls[s_cs[s][0] + (q - s_sego[s])];
+  R.s = s;
+  R.ip = ip;
+  const double *const *f = a.p.f;
+  R.x = f[PF_X][ip];
+  R.y = f[PF_Y][ip];
+  R.Mp = f[PF_MP][ip];
+  // momentum source: the DATA momentum on the first step, else velocity
+  R.m1 = a.mom_stored ? f[PF_MOMX][ip] : f[PF_VPX][ip];
+  R.m2 = a.mom_stored ? f[PF_MOMY][ip] : f[PF_VPY][ip];
+  R.Vp = f[PF_VP][ip];
+  R.hp = f[PF_HP][ip];
+  R.H = f[PF_H][ip];
+  R.frx = f[PF_FRX][ip];
+  R.fry = f[PF_FRY][ip];
+  R.f11 = f[PF_F11][ip];
+  R.f12 = f[PF_F12][ip];
+  R.f22 = f[PF_F22][ip];
 }
 
 __global__ void __launch_bounds__(kP2GThreads)
@@ -204,22 +261,22 @@ __global__ void __launch_bounds__(kP2GThreads)
   const int nrows = P.nrows, ncols = P.ncols;
   const double hx = P.hx, hy = P.hy;
   const int t = threadIdx.x;
-  const int c0 = blockIdx.x * kTile, r0 = blockIdx.y * kTile;
-  const int lr = t % kTile, lc = t / kTile;
+  const int c0 = blockIdx.x * kTC, r0 = blockIdx.y * kTR;
+  const int lr = t % kTR, lc = t / kTR;
   const int c = c0 + lc, r = r0 + lr;
   const bool node_ok = c <= ncols && r <= nrows;
   const int bid = blockIdx.y * gridDim.x + blockIdx.x;
 
-  __shared__ int s_cs[kSeg][kTile + 2];  // cstart, cell rows rl..rh+1
-  __shared__ int s_sego[kSeg + 1];       // tile-list offset per column
-  __shared__ double s_Z[(kTile + 2) * (kTile + 2)];
+  __shared__ int s_cs[kSeg][kTR + 2];  // cstart, cell rows rl..rh+1
+  __shared__ int s_sego[kSeg + 1];     // tile-list offset per column
+  __shared__ double s_Z[(kTC + 2) * (kTR + 2)];
   __shared__ double s_p[kNS][kChunk];
   __shared__ double s_red[6][kP2GThreads / 64];
 
   const int ccl = c0 - 1 < 0 ? 0 : c0 - 1;
-  const int cch = c0 + kTile - 1 > ncols - 1 ? ncols - 1 : c0 + kTile - 1;
+  const int cch = c0 + kTC - 1 > ncols - 1 ? ncols - 1 : c0 + kTC - 1;
   const int rl = r0 - 1 < 0 ? 0 : r0 - 1;
-  const int rh = r0 + kTile - 1 > nrows - 1 ? nrows - 1 : r0 + kTile - 1;
+  const int rh = r0 + kTR - 1 > nrows - 1 ? nrows - 1 : r0 + kTR - 1;
   const int nseg = cch - ccl + 1, nr = rh - rl + 1;
 
   for (int i = t; i < nseg * (nr + 1); i += kP2GThreads) {
@@ -243,9 +300,12 @@ __global__ void __launch_bounds__(kP2GThreads)
          Fix = 0, Fiy = 0;
 
   if (L > 0) {
-    // Z of nodes (c0-1 .. c0+16, r0-1 .. r0+16)
-    for (int i = t; i < (kTile + 2) * (kTile + 2); i += kP2GThreads) {
-      const int zc = c0 - 1 + i / (kTile + 2), zr = r0 - 1 + i % (kTile + 2);
+    // first chunk's gathers go out before the Z tile is staged
+    P2GRaw R;
+    if (t < L) p2g_fetch(a, t, nseg, s_cs, s_sego, R);
+    // Z of nodes (c0-1 .. c0+kTC, r0-1 .. r0+kTR)
+    for (int i = t; i < (kTC + 2) * (kTR + 2); i += kP2GThreads) {
+      const int zc = c0 - 1 + i / (kTR + 2), zr = r0 - 1 + i % (kTR + 2);
       s_Z[i] = (zc >= 0 && zc <= ncols && zr >= 0 && zr <= nrows)
                    ? a.Z[zr + zc * (nrows + 1)]
                    : 0.0;
@@ -266,24 +326,18 @@ __global__ void __launch_bounds__(kP2GThreads)
     __syncthreads();
 
     for (int base = 0; base < L; base += kChunk) {
-      const int q = base + t;
-      if (q < L) {
-        int s = 0;
-        while (s + 1 < nseg && s_sego[s + 1] <= q) ++s;
-        const int jp = s_cs[s][0] + (q - s_sego[s]);
-        const int ip = a.cptcls[jp];
-        const double x = a.p.f[PF_X][ip], y = a.p.f[PF_Y][ip];
+      if (base + t < L) {
         // The CSR segment fixes the cell column; the row is the binning's
         // floor(y/hy), clamped to the staged rows so LDS indices stay in
         // bounds whatever the input.
-        const int pc = ccl + s;
-        int pr = cell_of(x, y, nrows, ncols, hx, hy) % nrows;
+        const int pc = ccl + R.s;
+        int pr = cell_of(R.x, R.y, nrows, ncols, hx, hy) % nrows;
         pr = pr < rl ? rl : (pr > rh ? rh : pr);
-        const Local Lc = local_coords(x, y, pc, pr, hx, hy);
+        const Local Lc = local_coords(R.x, R.y, pc, pr, hx, hy);
         // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
-        const int zb = (pc - (c0 - 1)) * (kTile + 2) + (pr - (r0 - 1));
+        const int zb = (pc - (c0 - 1)) * (kTR + 2) + (pr - (r0 - 1));
         const double z0 = s_Z[zb], z1 = s_Z[zb + 1];
-        const double z2 = s_Z[zb + (kTile + 2)], z3 = s_Z[zb + (kTile + 3)];
+        const double z2 = s_Z[zb + (kTR + 2)], z3 = s_Z[zb + (kTR + 3)];
         double dzx = 0.0, dzy = 0.0;
         dzx += shape_dx(Lc, 0, hx) * z0;
         dzy += shape_dy(Lc, 0, hy) * z0;
@@ -293,51 +347,46 @@ __global__ void __launch_bounds__(kP2GThreads)
         dzy += shape_dy(Lc, 2, hy) * z2;
         dzx += shape_dx(Lc, 3, hx) * z3;
         dzy += shape_dy(Lc, 3, hy) * z3;
-        const double hp = a.p.f[PF_HP][ip], H = a.p.f[PF_H][ip];
-        const double Vp = a.p.f[PF_VP][ip], Mp = a.p.f[PF_MP][ip];
         // gravity along the slope with the well-balance factor,
         // optim_par.cpp:450-453
-        const double wb = hp > 1e-10 ? 1.0 - (H / hp) : 1.0;
-        const double Fpx = -P.g * Vp * P.rho * dzx * wb;
-        const double Fpy = -P.g * Vp * P.rho * dzy * wb;
-        double mx, my;
-        if (a.mom_stored) {
-          mx = a.p.f[PF_MOMX][ip];
-          my = a.p.f[PF_MOMY][ip];
-        } else {
-          // mom_p = vpx * Mp (optim_par.cpp:600-601)
-          mx = a.p.f[PF_VPX][ip] * Mp;
-          my = a.p.f[PF_VPY][ip] * Mp;
-        }
+        const double wb = R.hp > 1e-10 ? 1.0 - (R.H / R.hp) : 1.0;
+        const double Fpx = -P.g * R.Vp * P.rho * dzx * wb;
+        const double Fpy = -P.g * R.Vp * P.rho * dzy * wb;
+        // mom_p = vpx * Mp (optim_par.cpp:600-601), DATA momentum at step 1
+        const double mx = a.mom_stored ? R.m1 : R.m1 * R.Mp;
+        const double my = a.mom_stored ? R.m2 : R.m2 * R.Mp;
         s_p[0][t] = Lc.xi;
         s_p[1][t] = Lc.eta;
-        s_p[2][t] = Mp;
+        s_p[2][t] = R.Mp;
         s_p[3][t] = mx;
         s_p[4][t] = my;
         s_p[5][t] = Fpx;
         s_p[6][t] = Fpy;
-        s_p[7][t] = a.p.f[PF_FRX][ip];
-        s_p[8][t] = a.p.f[PF_FRY][ip];
-        s_p[9][t] = a.p.f[PF_F11][ip];
-        s_p[10][t] = a.p.f[PF_F12][ip];
-        s_p[11][t] = a.p.f[PF_F22][ip];
-        s_p[12][t] = Vp;
-        const bool owned = pc >= c0 && pc < c0 + kTile && pr >= r0 && pr < r0 + kTile;
+        s_p[7][t] = R.frx;
+        s_p[8][t] = R.fry;
+        s_p[9][t] = R.f11;
+        s_p[10][t] = R.f12;
+        s_p[11][t] = R.f22;
+        s_p[12][t] = R.Vp;
+        const bool owned = pc >= c0 && pc < c0 + kTC && pr >= r0 && pr < r0 + kTR;
         if (owned) {
-          pm += Mp;
+          pm += R.Mp;
           ppx += mx;
           ppy += my;
-          if (a.aux) {
-            a.p.f[PF_DZX][ip] = dzx;
-            a.p.f[PF_DZY][ip] = dzy;
-            a.p.f[PF_FPX][ip] = Fpx;
-            a.p.f[PF_FPY][ip] = Fpy;
-            a.p.f[PF_MOMX][ip] = mx;
-            a.p.f[PF_MOMY][ip] = my;
+          if (a.aux >= AUX_FULL) {
+            a.p.f[PF_DZX][R.ip] = dzx;
+            a.p.f[PF_DZY][R.ip] = dzy;
+            a.p.f[PF_FPX][R.ip] = Fpx;
+            a.p.f[PF_FPY][R.ip] = Fpy;
+            a.p.f[PF_MOMX][R.ip] = mx;
+            a.p.f[PF_MOMY][R.ip] = my;
           }
         }
       }
       __syncthreads();
+      // next chunk's gathers are in flight while this chunk is folded
+      if (base + kChunk + t < L)
+        p2g_fetch(a, base + kChunk + t, nseg, s_cs, s_sego, R);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int lo = tb[k] > base ? tb[k] : base;
@@ -367,7 +416,7 @@ __global__ void __launch_bounds__(kP2GThreads)
     }
   }
 
-  if (node_ok && (L > 0 || a.aux)) {
+  if (node_ok && (L > 0 || a.aux >= AUX_FULL)) {
     const int n = r + c * (nrows + 1);
     const double dt = ctrl->dt;
     // force balance and nodal velocities, optim_par.cpp:467-497 (D1)
@@ -391,7 +440,7 @@ __global__ void __launch_bounds__(kP2GThreads)
     }
     a.n.f[NF_VVXL][n] = vxL;
     a.n.f[NF_VVYL][n] = vyL;
-    if (a.aux) {
+    if (a.aux >= AUX_FULL) {
       a.n.f[NF_AVX][n] = ax;
       a.n.f[NF_AVY][n] = ay;
       a.n.f[NF_MV][n] = Mv;
@@ -579,7 +628,7 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
     // friction force for the next P2G, optim_par.cpp:456-459
     a.p.f[PF_FRX][ip] = Ap * Fbx;
     a.p.f[PF_FRY][ip] = Ap * Fby;
-    if (a.aux) {
+    if (a.aux >= AUX_MS) {
       a.p.f[PF_AP][ip] = Ap;
       a.p.f[PF_FBX][ip] = Fbx;
       a.p.f[PF_FBY][ip] = Fby;
@@ -587,6 +636,8 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
       a.p.f[PF_VXDY][ip] = vxdy;
       a.p.f[PF_VYDX][ip] = vydx;
       a.p.f[PF_VYDY][ip] = vydy;
+    }
+    if (a.aux >= AUX_FULL) {
       double ax = 0.0, ay = 0.0, zp = 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -614,16 +665,29 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
     a.key_out[ip] = newkey;
     a.slot[ip] = sl;
   }
-  // CFL maxima (non-negative doubles order like their bit patterns)
+  // CFL maxima: wave, then block, then one atomic per block into one of
+  // kCflSlots slots (non-negative doubles order like their bit patterns)
+  __shared__ double s_cfl[3][4];
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     vmx = fmax(vmx, __shfl_xor(vmx, d));
     vmy = fmax(vmy, __shfl_xor(vmy, d));
     hmx = fmax(hmx, __shfl_xor(hmx, d));
   }
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    const int s = ctrl->it & 1;
-    unsigned long long *m = const_cast<unsigned long long *>(ctrl->cfl[s]);
+    s_cfl[0][w] = vmx;
+    s_cfl[1][w] = vmy;
+    s_cfl[2][w] = hmx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      vmx = fmax(vmx, s_cfl[0][k]);
+      vmy = fmax(vmy, s_cfl[1][k]);
+      hmx = fmax(hmx, s_cfl[2][k]);
+    }
+    unsigned long long *m = ctrl->cfl[ctrl->it & 1][blockIdx.x & (kCflSlots - 1)];
     atomicMax(&m[0], (unsigned long long)__double_as_longlong(vmx));
     atomicMax(&m[1], (unsigned long long)__double_as_longlong(vmy));
     atomicMax(&m[2], (unsigned long long)__double_as_longlong(hmx));
@@ -940,10 +1004,10 @@ int launch_step(Sim &s) {
                        ((s.it_host + 1) % s.p.ms_interval == 0);
   const bool ms_next = s.p.ms_on && s.p.ms_interval > 0 &&
                        ((s.it_host + 2) % s.p.ms_interval == 0);
-  const bool aux = s.aux_on || ms_next;
+  const int aux = s.aux_on ? AUX_FULL : (ms_next ? AUX_MS : AUX_NONE);
   {
     Timer tm(s, "prologue");
-    prologue_kernel<<<1, 1, 0, s.st>>>(s.kp, s.ctrl);
+    prologue_kernel<<<1, 64, 0, s.st>>>(s.kp, s.ctrl);
     AMSC_LAUNCH_CHECK();
   }
   if (ms_step) {
@@ -966,7 +1030,7 @@ int launch_step(Sim &s) {
     a.ctrl = s.ctrl;
     a.n = s.nod;
     a.partials = s.diag ? s.partials : nullptr;
-    a.aux = aux ? 1 : 0;
+    a.aux = aux;
     a.mom_stored = s.mom_stored ? 1 : 0;
     dim3 grid(s.nblk_x, s.nblk_y);
     p2g_nodal_kernel<<<grid, kP2GThreads, 0, s.st>>>(a);
@@ -995,7 +1059,7 @@ int launch_step(Sim &s) {
     a.avy = s.nod.f[NF_AVY];
     a.Z = s.Z;
     a.ctrl = s.ctrl;
-    a.aux = aux ? 1 : 0;
+    a.aux = aux;
     if (s.np > 0) {
       particle_kernel<<<grid_for(s.np, 256), 256, 0, s.st>>>(a);
       AMSC_LAUNCH_CHECK();
@@ -1008,8 +1072,8 @@ int launch_step(Sim &s) {
                                        s.cptcls, &s.ctrl->active, s.st))
       return rc;
   }
-  s.aux_valid = aux;
-  s.nodal_valid = aux;
+  s.aux_valid = aux == AUX_FULL;
+  s.nodal_valid = aux == AUX_FULL;
   s.it_host++;
   s.particle_steps += s.np;
   if (s.p.dt_fixed > 0.0 && s.it_host >= s.p.nsteps) s.finished = true;
@@ -1064,8 +1128,8 @@ int amsc_sim_create(const amsc_params *p, int np, const double *x,
   if (s->cap < np) s->cap = np;
   s->nn = (p->nrows + 1) * (p->ncols + 1);
   s->nc = p->nrows * p->ncols;
-  s->nblk_x = (p->ncols + 1 + kTile - 1) / kTile;
-  s->nblk_y = (p->nrows + 1 + kTile - 1) / kTile;
+  s->nblk_x = (p->ncols + 1 + kTC - 1) / kTC;
+  s->nblk_y = (p->nrows + 1 + kTR - 1) / kTR;
   auto bail = [&](int rc) {
     sim_free(s);
     return rc;
