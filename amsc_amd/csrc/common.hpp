@@ -102,6 +102,56 @@ __device__ __forceinline__ double shape_dy(const Local &L, int inode,
   }
 }
 
+// Basis-gradient factors 1/hx, -1/hx, 1/hy, -1/hy, computed once on the
+// host (IEEE division gives the same values the expressions of
+// gpu_kernels.cpp:43-59 produce in every thread).
+struct Recip {
+  double ix, nix, iy, niy;
+};
+inline Recip make_recip(double hx, double hy) {
+  return Recip{1.0 / hx, -1.0 / hx, 1.0 / hy, -1.0 / hy};
+}
+
+__device__ __forceinline__ double shape_dx(const Local &L, int inode,
+                                           const Recip &R) {
+  switch (inode) {
+  case 3: return R.ix * L.eta;
+  case 2: return R.ix * (1.0 - L.eta);
+  case 1: return R.nix * L.eta;
+  default: return R.nix * (1.0 - L.eta);
+  }
+}
+
+__device__ __forceinline__ double shape_dy(const Local &L, int inode,
+                                           const Recip &R) {
+  switch (inode) {
+  case 3: return L.xi * R.iy;
+  case 2: return L.xi * R.niy;
+  case 1: return (1.0 - L.xi) * R.iy;
+  default: return (1.0 - L.xi) * R.niy;
+  }
+}
+
+// Exact division of a non-negative int (< 2^31) by a runtime constant d:
+// q = (n * m) >> k with k = 31 + ceil(log2 d), m = ceil(2^k / d).  The
+// error term n*(m*d - 2^k)/(d*2^k) stays below 1/d, so the floor is exact.
+struct FastDiv {
+  unsigned long long m;
+  int k, d;
+};
+inline FastDiv make_fastdiv(int d) {
+  int l = 0;
+  while ((1ll << l) < d) ++l;
+  FastDiv f;
+  f.k = 31 + l;
+  f.m = (unsigned long long)((((unsigned __int128)1 << f.k) + d - 1) / d);
+  f.d = d;
+  return f;
+}
+__device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
+  return (int)(((unsigned long long)(unsigned)n * f.m) >> f.k);
+}
+
 // Cell of a point: floor(x/hx), floor(y/hy) clamped to the grid.
 __device__ __forceinline__ int cell_of(double x, double y, int nrows,
                                        int ncols, double hx, double hy) {
@@ -115,6 +165,9 @@ __device__ __forceinline__ int cell_of(double x, double y, int nrows,
 __device__ __forceinline__ unsigned long long lanemask_lt() {
   const unsigned lane = __lane_id();
   return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+__device__ __forceinline__ unsigned long long lanemask_le() {
+  return lanemask_lt() | (1ull << __lane_id());
 }
 
 }  // namespace amsc

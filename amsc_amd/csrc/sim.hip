@@ -111,6 +111,11 @@ constexpr int kSeg = kTC + 1;          // cell columns (incl. halo)
 constexpr int kP2GThreads = kTC * kTR; // one node per thread
 constexpr int kChunk = kP2GThreads;    // particles staged per round
 constexpr int kNS = 13;                // staged values per particle
+// Staged particle i lives at i + i/16 in each LDS array: the node threads of
+// a wavefront walk neighbouring cells in lockstep, i.e. at a stride of about
+// 4 particles, which without the pad maps 16 lanes onto one bank.
+constexpr int kChunkPad = kChunk + kChunk / 16;
+__device__ __forceinline__ int lds_slot(int i) { return i + (i >> 4); }
 
 // Which optional fields a step writes: none; what the next merge/split
 // reads (Ap, Fb, velocity gradients); every output field and the nodal set.
@@ -204,16 +209,27 @@ __global__ void init_kernel(int np, KParams P, PPtr p, const int *key,
   p.level[ip] = 0;
 }
 
+// Output-only destinations of p2g_nodal (read from device memory only on
+// steps that write them, which keeps the kernel's argument block small).
+struct P2GOut {
+  double *dzx, *dzy, *fpx, *fpy, *momx, *momy;  // particle fields
+  double *nod[NF_COUNT];                         // nodal set
+};
+
 struct P2GArgs {
-  KParams P;
+  int nrows, ncols, bc, aux, mom_stored;
+  double hx, hy, g, rho;
+  Recip rc;
   const int *cstart, *cptcls;
-  PPtr p;
+  // x, y, Mp, momentum source (vpx/vpy, or the DATA momentum on the first
+  // step), Vp, hp, H, friction force, stress
+  const double *x, *y, *Mp, *m1, *m2, *Vp, *hp, *H, *frx, *fry, *f11, *f12,
+      *f22;
   const double *Z;
   const Ctrl *ctrl;
-  NPtr n;
+  double *vvxL, *vvyL;
   double *partials;  // [nblocks][6] or null
-  int aux;           // AUX_FULL: avx/avy, nodal dump, P2G particle outputs
-  int mom_stored;    // momentum from the mom arrays (first step only)
+  const P2GOut *out;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -225,41 +241,45 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One particle's P2G inputs, gathered through cptcls.
 struct P2GRaw {
   double x, y, Mp, m1, m2, Vp, hp, H, frx, fry, f11, f12, f22;
-  int ip, s;
+};
+// Where a tile-list position lives: particle id, cell column segment and
+// cell row within the segment.
+struct P2GIdx {
+  int ip, s, j;
 };
 
-__device__ __forceinline__ void p2g_fetch(const P2GArgs &a, int q, int nseg,
-                                          const int (*s_cs)[kTR + 2],
-                                          const int *s_sego, P2GRaw &R) {
-  int s = 0;
-  while (s + 1 < nseg && s_sego[s + 1] <= q) ++s;
-  const int ip = a.cptcls[s_cs[s][0] + (q - s_sego[s])];
-  R.s = s;
-  R.ip = ip;
-  const double *const *f = a.p.f;
-  R.x = f[PF_X][ip];
-  R.y = f[PF_Y][ip];
-  R.Mp = f[PF_MP][ip];
-  // momentum source: the DATA momentum on the first step, else velocity
-  R.m1 = a.mom_stored ? f[PF_MOMX][ip] : f[PF_VPX][ip];
-  R.m2 = a.mom_stored ? f[PF_MOMY][ip] : f[PF_VPY][ip];
-  R.Vp = f[PF_VP][ip];
-  R.hp = f[PF_HP][ip];
-  R.H = f[PF_H][ip];
-  R.frx = f[PF_FRX][ip];
-  R.fry = f[PF_FRY][ip];
-  R.f11 = f[PF_F11][ip];
-  R.f12 = f[PF_F12][ip];
-  R.f22 = f[PF_F22][ip];
+__device__ __forceinline__ void p2g_fields(const P2GArgs &a, int ip,
+                                           P2GRaw &R) {
+  R.x = a.x[ip];
+  R.y = a.y[ip];
+  R.Mp = a.Mp[ip];
+  R.m1 = a.m1[ip];
+  R.m2 = a.m2[ip];
+  R.Vp = a.Vp[ip];
+  R.hp = a.hp[ip];
+  R.H = a.H[ip];
+  R.frx = a.frx[ip];
+  R.fry = a.fry[ip];
+  R.f11 = a.f11[ip];
+  R.f12 = a.f12[ip];
+  R.f22 = a.f22[ip];
 }
 
+// P2G + nodal update for one tile of kTC x kTR nodes.
+//
+// The tile's particles are the CSR ranges of its (kTC+1) x (kTR+1) cells,
+// taken column segment by column segment ("tile list"), and staged through
+// LDS kChunk at a time.  Each chunk goes through a two-stage pipeline: while
+// chunk k is folded, the fields of chunk k+1 and the particle ids of chunk
+// k+2 are in flight.  A per-chunk owner map (cell of every tile-list
+// position, built cell-parallel) turns a position into its particle id with
+// one LDS read.
 __global__ void __launch_bounds__(kP2GThreads)
     p2g_nodal_kernel(P2GArgs a) {
   const Ctrl *ctrl = a.ctrl;
   if (!ctrl->active) return;
-  const KParams &P = a.P;
-  const int nrows = P.nrows, ncols = P.ncols;
-  const double hx = P.hx, hy = P.hy;
+  const int nrows = a.nrows, ncols = a.ncols;
+  const double hx = a.hx, hy = a.hy;
   const int t = threadIdx.x;
   const int c0 = blockIdx.x * kTC, r0 = blockIdx.y * kTR;
   const int lr = t % kTR, lc = t / kTR;
@@ -270,7 +290,8 @@ __global__ void __launch_bounds__(kP2GThreads)
   __shared__ int s_cs[kSeg][kTR + 2];  // cstart, cell rows rl..rh+1
   __shared__ int s_sego[kSeg + 1];     // tile-list offset per column
   __shared__ double s_Z[(kTC + 2) * (kTR + 2)];
-  __shared__ double s_p[kNS][kChunk];
+  __shared__ double s_p[kNS][kChunkPad];
+  __shared__ unsigned short s_own[kChunk];
   __shared__ double s_red[6][kP2GThreads / 64];
 
   const int ccl = c0 - 1 < 0 ? 0 : c0 - 1;
@@ -295,14 +316,34 @@ __global__ void __launch_bounds__(kP2GThreads)
   __syncthreads();
   const int L = s_sego[nseg];
 
+  // owner map of the chunk starting at base (positions base .. base+kChunk)
+  auto build_own = [&](int base) {
+    if (base >= L) return;
+    for (int ci = t; ci < nseg * nr; ci += kP2GThreads) {
+      const int s = ci / nr, j = ci % nr;
+      const int b = s_sego[s] + (s_cs[s][j] - s_cs[s][0]);
+      const int e = b + (s_cs[s][j + 1] - s_cs[s][j]);
+      const int lo = b > base ? b : base;
+      const int hi = e < base + kChunk ? e : base + kChunk;
+      for (int q = lo; q < hi; ++q)
+        s_own[q - base] = (unsigned short)((s << 8) | j);
+    }
+  };
+  auto fetch_idx = [&](int q, P2GIdx &I) {
+    const int v = s_own[q & (kChunk - 1)];
+    I.s = v >> 8;
+    I.j = v & 0xFF;
+    I.ip = a.cptcls[s_cs[I.s][0] + (q - s_sego[I.s])];
+  };
+
   double pm = 0, ppx = 0, ppy = 0;  // particle sums (owned cells)
   double Mv = 0, mvx = 0, mvy = 0, FPx = 0, FPy = 0, Frx = 0, Fry = 0,
          Fix = 0, Fiy = 0;
 
   if (L > 0) {
-    // first chunk's gathers go out before the Z tile is staged
+    P2GIdx I1, I2, I3;
     P2GRaw R;
-    if (t < L) p2g_fetch(a, t, nseg, s_cs, s_sego, R);
+    build_own(0);
     // Z of nodes (c0-1 .. c0+kTC, r0-1 .. r0+kTR)
     for (int i = t; i < (kTC + 2) * (kTR + 2); i += kP2GThreads) {
       const int zc = c0 - 1 + i / (kTR + 2), zr = r0 - 1 + i % (kTR + 2);
@@ -324,76 +365,82 @@ __global__ void __launch_bounds__(kP2GThreads)
       }
     }
     __syncthreads();
+    if (t < L) fetch_idx(t, I1);
+    __syncthreads();
+    build_own(kChunk);
+    __syncthreads();
+    if (kChunk + t < L) fetch_idx(kChunk + t, I2);
+    if (t < L) p2g_fields(a, I1.ip, R);
+    __syncthreads();  // s_own is read above and rebuilt in the loop
 
     for (int base = 0; base < L; base += kChunk) {
       if (base + t < L) {
-        // The CSR segment fixes the cell column; the row is the binning's
-        // floor(y/hy), clamped to the staged rows so LDS indices stay in
-        // bounds whatever the input.
-        const int pc = ccl + R.s;
-        int pr = cell_of(R.x, R.y, nrows, ncols, hx, hy) % nrows;
-        pr = pr < rl ? rl : (pr > rh ? rh : pr);
+        // the particle's cell is the CSR cell it was found in
+        const int pc = ccl + I1.s, pr = rl + I1.j;
         const Local Lc = local_coords(R.x, R.y, pc, pr, hx, hy);
         // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
         const int zb = (pc - (c0 - 1)) * (kTR + 2) + (pr - (r0 - 1));
         const double z0 = s_Z[zb], z1 = s_Z[zb + 1];
         const double z2 = s_Z[zb + (kTR + 2)], z3 = s_Z[zb + (kTR + 3)];
         double dzx = 0.0, dzy = 0.0;
-        dzx += shape_dx(Lc, 0, hx) * z0;
-        dzy += shape_dy(Lc, 0, hy) * z0;
-        dzx += shape_dx(Lc, 1, hx) * z1;
-        dzy += shape_dy(Lc, 1, hy) * z1;
-        dzx += shape_dx(Lc, 2, hx) * z2;
-        dzy += shape_dy(Lc, 2, hy) * z2;
-        dzx += shape_dx(Lc, 3, hx) * z3;
-        dzy += shape_dy(Lc, 3, hy) * z3;
+        dzx += shape_dx(Lc, 0, a.rc) * z0;
+        dzy += shape_dy(Lc, 0, a.rc) * z0;
+        dzx += shape_dx(Lc, 1, a.rc) * z1;
+        dzy += shape_dy(Lc, 1, a.rc) * z1;
+        dzx += shape_dx(Lc, 2, a.rc) * z2;
+        dzy += shape_dy(Lc, 2, a.rc) * z2;
+        dzx += shape_dx(Lc, 3, a.rc) * z3;
+        dzy += shape_dy(Lc, 3, a.rc) * z3;
         // gravity along the slope with the well-balance factor,
         // optim_par.cpp:450-453
         const double wb = R.hp > 1e-10 ? 1.0 - (R.H / R.hp) : 1.0;
-        const double Fpx = -P.g * R.Vp * P.rho * dzx * wb;
-        const double Fpy = -P.g * R.Vp * P.rho * dzy * wb;
+        const double Fpx = -a.g * R.Vp * a.rho * dzx * wb;
+        const double Fpy = -a.g * R.Vp * a.rho * dzy * wb;
         // mom_p = vpx * Mp (optim_par.cpp:600-601), DATA momentum at step 1
         const double mx = a.mom_stored ? R.m1 : R.m1 * R.Mp;
         const double my = a.mom_stored ? R.m2 : R.m2 * R.Mp;
-        s_p[0][t] = Lc.xi;
-        s_p[1][t] = Lc.eta;
-        s_p[2][t] = R.Mp;
-        s_p[3][t] = mx;
-        s_p[4][t] = my;
-        s_p[5][t] = Fpx;
-        s_p[6][t] = Fpy;
-        s_p[7][t] = R.frx;
-        s_p[8][t] = R.fry;
-        s_p[9][t] = R.f11;
-        s_p[10][t] = R.f12;
-        s_p[11][t] = R.f22;
-        s_p[12][t] = R.Vp;
+        const int ts = lds_slot(t);
+        s_p[0][ts] = Lc.xi;
+        s_p[1][ts] = Lc.eta;
+        s_p[2][ts] = R.Mp;
+        s_p[3][ts] = mx;
+        s_p[4][ts] = my;
+        s_p[5][ts] = Fpx;
+        s_p[6][ts] = Fpy;
+        s_p[7][ts] = R.frx;
+        s_p[8][ts] = R.fry;
+        s_p[9][ts] = R.f11;
+        s_p[10][ts] = R.f12;
+        s_p[11][ts] = R.f22;
+        s_p[12][ts] = R.Vp;
         const bool owned = pc >= c0 && pc < c0 + kTC && pr >= r0 && pr < r0 + kTR;
         if (owned) {
           pm += R.Mp;
           ppx += mx;
           ppy += my;
           if (a.aux >= AUX_FULL) {
-            a.p.f[PF_DZX][R.ip] = dzx;
-            a.p.f[PF_DZY][R.ip] = dzy;
-            a.p.f[PF_FPX][R.ip] = Fpx;
-            a.p.f[PF_FPY][R.ip] = Fpy;
-            a.p.f[PF_MOMX][R.ip] = mx;
-            a.p.f[PF_MOMY][R.ip] = my;
+            const P2GOut *o = a.out;
+            o->dzx[I1.ip] = dzx;
+            o->dzy[I1.ip] = dzy;
+            o->fpx[I1.ip] = Fpx;
+            o->fpy[I1.ip] = Fpy;
+            o->momx[I1.ip] = mx;
+            o->momy[I1.ip] = my;
           }
         }
       }
+      build_own(base + 2 * kChunk);
       __syncthreads();
-      // next chunk's gathers are in flight while this chunk is folded
-      if (base + kChunk + t < L)
-        p2g_fetch(a, base + kChunk + t, nseg, s_cs, s_sego, R);
+      // in flight during the fold: ids of chunk k+2, fields of chunk k+1
+      if (base + 2 * kChunk + t < L) fetch_idx(base + 2 * kChunk + t, I3);
+      if (base + kChunk + t < L) p2g_fields(a, I2.ip, R);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int lo = tb[k] > base ? tb[k] : base;
         const int hi = te[k] < base + kChunk ? te[k] : base + kChunk;
         const int corner = 3 - k;
         for (int qq = lo; qq < hi; ++qq) {
-          const int i = qq - base;
+          const int i = lds_slot(qq - base);
           Local Lq;
           Lq.xi = s_p[0][i];
           Lq.eta = s_p[1][i];
@@ -405,14 +452,16 @@ __global__ void __launch_bounds__(kP2GThreads)
           FPy += N * s_p[6][i];
           Frx += N * s_p[7][i];
           Fry += N * s_p[8][i];
-          const double Nx = shape_dx(Lq, corner, hx);
-          const double Ny = shape_dy(Lq, corner, hy);
+          const double Nx = shape_dx(Lq, corner, a.rc);
+          const double Ny = shape_dy(Lq, corner, a.rc);
           const double vp = s_p[12][i];
           Fix += (Nx * s_p[9][i] + Ny * s_p[10][i]) * vp;
           Fiy += (Nx * s_p[10][i] + Ny * s_p[11][i]) * vp;
         }
       }
       __syncthreads();
+      I1 = I2;
+      I2 = I3;
     }
   }
 
@@ -430,7 +479,7 @@ __global__ void __launch_bounds__(kP2GThreads)
     double vxL = dt * ax + vx, vyL = dt * ay + vy;
     // boundary conditions, optim_par.cpp:503-529 (nodes of the first/last
     // cell column: x components; first/last cell row: y components)
-    if (P.bc) {
+    if (a.bc) {
       if (c == 0 || c == 1 || c == ncols - 1 || c == ncols) {
         ax = 0.0; vx = 0.0; vxL = 0.0; mx = 0.0;
       }
@@ -438,26 +487,27 @@ __global__ void __launch_bounds__(kP2GThreads)
         ay = 0.0; vy = 0.0; vyL = 0.0; my = 0.0;
       }
     }
-    a.n.f[NF_VVXL][n] = vxL;
-    a.n.f[NF_VVYL][n] = vyL;
+    a.vvxL[n] = vxL;
+    a.vvyL[n] = vyL;
     if (a.aux >= AUX_FULL) {
-      a.n.f[NF_AVX][n] = ax;
-      a.n.f[NF_AVY][n] = ay;
-      a.n.f[NF_MV][n] = Mv;
-      a.n.f[NF_MOMX][n] = mx;
-      a.n.f[NF_MOMY][n] = my;
-      a.n.f[NF_FEXTX][n] = Fex;
-      a.n.f[NF_FEXTY][n] = Fey;
-      a.n.f[NF_FINTX][n] = Fix;
-      a.n.f[NF_FINTY][n] = Fiy;
-      a.n.f[NF_FRX][n] = Frx;
-      a.n.f[NF_FRY][n] = Fry;
-      a.n.f[NF_FPX][n] = FPx;
-      a.n.f[NF_FPY][n] = FPy;
-      a.n.f[NF_FTOTX][n] = Ftx;
-      a.n.f[NF_FTOTY][n] = Fty;
-      a.n.f[NF_VVX][n] = vx;
-      a.n.f[NF_VVY][n] = vy;
+      double *const *nod = a.out->nod;
+      nod[NF_AVX][n] = ax;
+      nod[NF_AVY][n] = ay;
+      nod[NF_MV][n] = Mv;
+      nod[NF_MOMX][n] = mx;
+      nod[NF_MOMY][n] = my;
+      nod[NF_FEXTX][n] = Fex;
+      nod[NF_FEXTY][n] = Fey;
+      nod[NF_FINTX][n] = Fix;
+      nod[NF_FINTY][n] = Fiy;
+      nod[NF_FRX][n] = Frx;
+      nod[NF_FRY][n] = Fry;
+      nod[NF_FPX][n] = FPx;
+      nod[NF_FPY][n] = FPy;
+      nod[NF_FTOTX][n] = Ftx;
+      nod[NF_FTOTY][n] = Fty;
+      nod[NF_VVX][n] = vx;
+      nod[NF_VVY][n] = vy;
     }
   }
 
@@ -468,14 +518,14 @@ __global__ void __launch_bounds__(kP2GThreads)
     const int lane = t & 63, w = t >> 6;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const double s = wave_sum(v[k]);
-      if (lane == 0) s_red[k][w] = s;
+      const double sm = wave_sum(v[k]);
+      if (lane == 0) s_red[k][w] = sm;
     }
     __syncthreads();
     if (t < 6) {
-      double s = 0.0;
-      for (int ww = 0; ww < kP2GThreads / 64; ++ww) s += s_red[t][ww];
-      a.partials[bid * 6 + t] = s;
+      double sm = 0.0;
+      for (int ww = 0; ww < kP2GThreads / 64; ++ww) sm += s_red[t][ww];
+      a.partials[bid * 6 + t] = sm;
     }
   }
 }
@@ -507,6 +557,8 @@ __global__ void conservation_kernel(const double *partials, int nb,
 
 struct PartArgs {
   KParams P;
+  Recip rc;
+  FastDiv fd;  // division by nrows
   int np;
   PPtr p;
   const int *key_in;  // step-start cell of each particle
@@ -517,7 +569,9 @@ struct PartArgs {
   int aux;
 };
 
-__global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
+template <int AUX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+particle_kernel(PartArgs a) {
   Ctrl *ctrl = a.ctrl;
   const KParams &P = a.P;
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
@@ -533,7 +587,7 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
   if (valid) {
     const int nrows = P.nrows;
     const double hx = P.hx, hy = P.hy;
-    const int cell = a.key_in[ip], r = cell % nrows, c = cell / nrows;
+    const int cell = a.key_in[ip], c = fdiv(cell, a.fd), r = cell - c * nrows;
     int nid[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) nid[k] = node_id(k, c, r, nrows);
@@ -567,7 +621,7 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
     double vxdx = 0.0, vxdy = 0.0, vydx = 0.0, vydy = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const double Nx = shape_dx(L1, k, hx), Ny = shape_dy(L1, k, hy);
+      const double Nx = shape_dx(L1, k, a.rc), Ny = shape_dy(L1, k, a.rc);
       vxdx += Nx * gx[k];
       vxdy += Ny * gx[k];
       vydx += Nx * gy[k];
@@ -597,9 +651,18 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
       const double nrm = sqrt(vpx * vpx + vpy * vpy);
       const double ALF = h > 1.e-3 ? (6. * mu * nrm) / ((h + 0.001) * tau_Y) : 0.0;
       const double B = -114. / 32. - ALF;
-      const double Z1 = (-B + sqrt(B * B - 4. * A * C)) / (2. * A);
-      const double Z2 = (-B - sqrt(B * B - 4. * A * C)) / (2. * A);
-      const double ZZ = fabs(Z1 - .5) <= .5 ? Z1 : Z2;
+      // Z1 = (-B + sq)/(2A) is selected only if it lies in [0, 1]; when
+      // -B + sq > 3.0000001 it cannot (2A = 3), so Z2 is taken without
+      // evaluating Z1.  Same values as computing both.
+      const double sq = sqrt(B * B - 4. * A * C);
+      const double num1 = -B + sq;
+      double ZZ;
+      if (num1 > 3.0000001) {
+        ZZ = (-B - sq) / (2. * A);
+      } else {
+        const double Z1 = num1 / (2. * A);
+        ZZ = fabs(Z1 - .5) <= .5 ? Z1 : (-B - sq) / (2. * A);
+      }
       const double D_xx = vxdx;
       const double D_xy = 0.5 * (vxdy + vydx);
       const double D_yy = vydy;
@@ -628,7 +691,7 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
     // friction force for the next P2G, optim_par.cpp:456-459
     a.p.f[PF_FRX][ip] = Ap * Fbx;
     a.p.f[PF_FRY][ip] = Ap * Fby;
-    if (a.aux >= AUX_MS) {
+    if (AUX >= AUX_MS) {
       a.p.f[PF_AP][ip] = Ap;
       a.p.f[PF_FBX][ip] = Fbx;
       a.p.f[PF_FBY][ip] = Fby;
@@ -637,7 +700,7 @@ __global__ void __launch_bounds__(256) particle_kernel(PartArgs a) {
       a.p.f[PF_VYDX][ip] = vydx;
       a.p.f[PF_VYDY][ip] = vydy;
     }
-    if (a.aux >= AUX_FULL) {
+    if (AUX >= AUX_FULL) {
       double ax = 0.0, ay = 0.0, zp = 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -806,6 +869,7 @@ struct Sim {
   Ctrl *ctrl = nullptr;
   Ctrl *ctrl_host = nullptr;  // pinned
   double *partials = nullptr;
+  P2GOut *p2g_out[2] = {nullptr, nullptr};  // device copies, one per buffer
   int nblk_x = 0, nblk_y = 0;
   // merge/split workspace
   int *act = nullptr, *partner = nullptr, *wts = nullptr, *offs = nullptr;
@@ -900,6 +964,8 @@ void sim_free(Sim *s) {
   if (s->ctrl) (void)hipFree(s->ctrl);
   if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
   if (s->partials) (void)hipFree(s->partials);
+  for (int b = 0; b < 2; ++b)
+    if (s->p2g_out[b]) (void)hipFree(s->p2g_out[b]);
   for (auto e : s->event_pool) (void)hipEventDestroy(e);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
@@ -1022,16 +1088,37 @@ int launch_step(Sim &s) {
   {
     Timer tm(s, "p2g_nodal");
     P2GArgs a;
-    a.P = s.kp;
-    a.cstart = s.bin.cstart;
-    a.cptcls = s.cptcls;
-    a.p = p;
-    a.Z = s.Z;
-    a.ctrl = s.ctrl;
-    a.n = s.nod;
-    a.partials = s.diag ? s.partials : nullptr;
+    a.nrows = s.p.nrows;
+    a.ncols = s.p.ncols;
+    a.bc = s.p.bc_flag;
     a.aux = aux;
     a.mom_stored = s.mom_stored ? 1 : 0;
+    a.hx = s.p.hx;
+    a.hy = s.p.hy;
+    a.g = s.p.g;
+    a.rho = s.p.rho;
+    a.rc = make_recip(s.p.hx, s.p.hy);
+    a.cstart = s.bin.cstart;
+    a.cptcls = s.cptcls;
+    a.x = p.f[PF_X];
+    a.y = p.f[PF_Y];
+    a.Mp = p.f[PF_MP];
+    a.m1 = s.mom_stored ? p.f[PF_MOMX] : p.f[PF_VPX];
+    a.m2 = s.mom_stored ? p.f[PF_MOMY] : p.f[PF_VPY];
+    a.Vp = p.f[PF_VP];
+    a.hp = p.f[PF_HP];
+    a.H = p.f[PF_H];
+    a.frx = p.f[PF_FRX];
+    a.fry = p.f[PF_FRY];
+    a.f11 = p.f[PF_F11];
+    a.f12 = p.f[PF_F12];
+    a.f22 = p.f[PF_F22];
+    a.Z = s.Z;
+    a.ctrl = s.ctrl;
+    a.vvxL = s.nod.f[NF_VVXL];
+    a.vvyL = s.nod.f[NF_VVYL];
+    a.partials = s.diag ? s.partials : nullptr;
+    a.out = s.p2g_out[s.cur];
     dim3 grid(s.nblk_x, s.nblk_y);
     p2g_nodal_kernel<<<grid, kP2GThreads, 0, s.st>>>(a);
     AMSC_LAUNCH_CHECK();
@@ -1047,6 +1134,8 @@ int launch_step(Sim &s) {
     Timer tm(s, "particle");
     PartArgs a;
     a.P = s.kp;
+    a.rc = make_recip(s.p.hx, s.p.hy);
+    a.fd = make_fastdiv(s.p.nrows);
     a.np = s.np;
     a.p = p;
     a.key_in = s.key[s.kcur];
@@ -1061,7 +1150,10 @@ int launch_step(Sim &s) {
     a.ctrl = s.ctrl;
     a.aux = aux;
     if (s.np > 0) {
-      particle_kernel<<<grid_for(s.np, 256), 256, 0, s.st>>>(a);
+      const unsigned g = grid_for(s.np, 256);
+      if (aux == AUX_FULL) particle_kernel<AUX_FULL><<<g, 256, 0, s.st>>>(a);
+      else if (aux == AUX_MS) particle_kernel<AUX_MS><<<g, 256, 0, s.st>>>(a);
+      else particle_kernel<AUX_NONE><<<g, 256, 0, s.st>>>(a);
       AMSC_LAUNCH_CHECK();
     }
   }
@@ -1156,8 +1248,22 @@ int amsc_sim_create(const amsc_params *p, int np, const double *x,
   ok &= hipHostMalloc(&s->ctrl_host, sizeof(Ctrl)) == hipSuccess;
   ok &= hipMalloc(&s->partials,
                   sizeof(double) * 6 * s->nblk_x * s->nblk_y) == hipSuccess;
+  for (int b = 0; b < 2; ++b)
+    ok &= hipMalloc(&s->p2g_out[b], sizeof(P2GOut)) == hipSuccess;
   if (!ok) return bail(fail(AMSC_ERR_NOMEM, "amsc_sim_create: allocation failed"));
   if (int rc = binning_alloc(s->bin, p->nrows, p->ncols)) return bail(rc);
+  for (int b = 0; b < 2; ++b) {
+    P2GOut o;
+    o.dzx = s->buf[b].f[PF_DZX];
+    o.dzy = s->buf[b].f[PF_DZY];
+    o.fpx = s->buf[b].f[PF_FPX];
+    o.fpy = s->buf[b].f[PF_FPY];
+    o.momx = s->buf[b].f[PF_MOMX];
+    o.momy = s->buf[b].f[PF_MOMY];
+    for (int k = 0; k < NF_COUNT; ++k) o.nod[k] = s->nod.f[k];
+    if (hipMemcpy(s->p2g_out[b], &o, sizeof(o), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(AMSC_ERR_HIP, "upload failed"));
+  }
 
   hipStream_t st = s->st;
   PPtr &b = s->buf[0];
