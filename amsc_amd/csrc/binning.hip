@@ -141,13 +141,14 @@ __global__ void key_count_kernel(int np, const double *__restrict__ x,
   }
 }
 
-__global__ void place_kernel(int np, const int *__restrict__ key,
+__global__ void place_kernel(int np, const int *__restrict__ np_dev,
+                             const int *__restrict__ key,
                              const int *__restrict__ slot,
                              const int *__restrict__ cstart, int *cptcls,
                              const int *__restrict__ active) {
   if (active && !*active) return;
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= np) return;
+  if (ip >= (np_dev ? *np_dev : np)) return;
   cptcls[cstart[key[ip]] + slot[ip]] = ip;
 }
 
@@ -205,6 +206,18 @@ __global__ void colouring_kernel(int nrows, int ncols, int *ccidx, int o1,
 
 }  // namespace
 
+int scan_reserve(ScanWork &w, int n) {
+  const int nb = (int)((n + (long long)kScanChunk - 1) / kScanChunk);
+  if (w.nb_cap >= nb + 1) return AMSC_OK;
+  if (w.bsum) (void)hipFree(w.bsum);
+  w.bsum = nullptr;
+  w.nb_cap = 0;
+  if (hipMalloc(&w.bsum, sizeof(int) * (nb + 1)) != hipSuccess)
+    return fail(AMSC_ERR_NOMEM, "scan workspace allocation failed");
+  w.nb_cap = nb + 1;
+  return AMSC_OK;
+}
+
 int exclusive_scan(const int *in, int *out, int n, bool zero_in, ScanWork &w,
                    hipStream_t st, int *total_host, const int *active) {
   if (n <= 0) {
@@ -217,13 +230,7 @@ int exclusive_scan(const int *in, int *out, int n, bool zero_in, ScanWork &w,
     return AMSC_OK;
   }
   const int nb = (int)((n + (long long)kScanChunk - 1) / kScanChunk);
-  if (w.nb_cap < nb + 1) {
-    if (w.bsum) (void)hipFree(w.bsum);
-    w.bsum = nullptr;
-    if (hipMalloc(&w.bsum, sizeof(int) * (nb + 1)) != hipSuccess)
-      return fail(AMSC_ERR_NOMEM, "scan workspace allocation failed");
-    w.nb_cap = nb + 1;
-  }
+  if (int rc = scan_reserve(w, n)) return rc;
   scan_partials_kernel<<<nb, kScanBlock, 0, st>>>(in, n, w.bsum, active);
   scan_bsums_kernel<<<1, kScanBlock, 0, st>>>(w.bsum, nb, active);
   scan_down_kernel<<<nb, kScanBlock, 0, st>>>(const_cast<int *>(in), out, n,
@@ -268,14 +275,14 @@ int launch_key_count(int np, const double *x, const double *y, int nrows,
 
 int launch_finish_binning(Binning &b, int np, const int *key,
                           const int *slot, int *cptcls, const int *active,
-                          hipStream_t st) {
+                          hipStream_t st, const int *np_dev) {
   const int nc = b.nrows * b.ncols;
   if (int rc = exclusive_scan(b.count, b.cstart, nc, true, b.scan, st,
                               nullptr, active))
     return rc;
   if (np > 0) {
-    place_kernel<<<grid_for(np, 256), 256, 0, st>>>(np, key, slot, b.cstart,
-                                                    cptcls, active);
+    place_kernel<<<grid_for(np, 256), 256, 0, st>>>(np, np_dev, key, slot,
+                                                    b.cstart, cptcls, active);
     cell_sort_kernel<<<grid_for(nc, 256), 256, 0, st>>>(nc, b.cstart, cptcls,
                                                         active);
     AMSC_LAUNCH_CHECK();

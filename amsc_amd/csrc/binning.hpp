@@ -19,6 +19,10 @@ int exclusive_scan(const int *in, int *out, int n, bool zero_in,
                    ScanWork &w, hipStream_t st, int *total_host,
                    const int *active = nullptr);
 
+// Size the workspace for scans of up to n values (a scan of a size already
+// reserved allocates nothing, so it may be captured into a graph).
+int scan_reserve(ScanWork &w, int n);
+
 // Device binning pipeline state for one particle set.
 struct Binning {
   int nrows = 0, ncols = 0;
@@ -35,10 +39,11 @@ int launch_key_count(int np, const double *x, const double *y, int nrows,
                      int ncols, double hx, double hy, int *key, int *slot,
                      int *count, const int *active, hipStream_t st);
 // cstart from counts, then cptcls from (key, slot), then ascending particle
-// ids within every cell.
+// ids within every cell.  With np_dev non-null the particle count is read
+// from device memory (np then only sizes the launch and must bound it).
 int launch_finish_binning(Binning &b, int np, const int *key,
                           const int *slot, int *cptcls, const int *active,
-                          hipStream_t st);
+                          hipStream_t st, const int *np_dev = nullptr);
 // ccidx for the colouring colour = (r&1) + 2*(c&1); coloff on the host.
 int launch_colouring(int nrows, int ncols, int *ccidx, int *coloff,
                      hipStream_t st);

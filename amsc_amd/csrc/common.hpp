@@ -132,6 +132,50 @@ __device__ __forceinline__ double shape_dy(const Local &L, int inode,
   }
 }
 
+// The basis in an arithmetic type R (the precision-generic time loop).  For
+// R = double these are the expressions above, value for value.
+template <typename R> struct RecipT {
+  R ix, nix, iy, niy;
+};
+template <typename R> inline RecipT<R> make_recip_t(double hx, double hy) {
+  return RecipT<R>{(R)(1.0 / hx), (R)(-1.0 / hx), (R)(1.0 / hy),
+                   (R)(-1.0 / hy)};
+}
+
+template <typename R>
+__device__ __forceinline__ R shape_t(R xi, R eta, int inode) {
+  switch (inode) {
+  case 3: return xi * eta;
+  case 2: return xi * (R(1) - eta);
+  case 1: return (R(1) - xi) * eta;
+  default: return (R(1) - xi) * (R(1) - eta);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R shape_dx_t(R xi, R eta, int inode,
+                                        const RecipT<R> &rc) {
+  (void)xi;
+  switch (inode) {
+  case 3: return rc.ix * eta;
+  case 2: return rc.ix * (R(1) - eta);
+  case 1: return rc.nix * eta;
+  default: return rc.nix * (R(1) - eta);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R shape_dy_t(R xi, R eta, int inode,
+                                        const RecipT<R> &rc) {
+  (void)eta;
+  switch (inode) {
+  case 3: return xi * rc.iy;
+  case 2: return xi * rc.niy;
+  case 1: return (R(1) - xi) * rc.iy;
+  default: return (R(1) - xi) * rc.niy;
+  }
+}
+
 // Exact division of a non-negative int (< 2^31) by a runtime constant d:
 // q = (n * m) >> k with k = 31 + ceil(log2 d), m = ceil(2^k / d).  The
 // error term n*(m*d - 2^k)/(d*2^k) stays below 1/d, so the floor is exact.

@@ -23,94 +23,8 @@ __global__ void ms_flag_kernel(int np, MsParams m, const double *dt_dev,
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np) return;
   const double dt = dt_dev ? *dt_dev : dt_host;
-  const double r_i = sqrt(in.Ap[ip]);
-  const double exx = in.vxdx[ip], eyy = in.vydy[ip];
-  const double exy = 0.5 * (in.vxdy[ip] + in.vydx[ip]);
-  const double gdot = sqrt(2.0 * (exx * exx + eyy * eyy + 2.0 * exy * exy));
-  const bool stretched = r_i > m.r_split;
-  const bool sheared = gdot * dt > m.shear_split;
-  const bool thick = r_i < m.r_merge;
-  const bool thin_skip = in.hp[ip] < m.split_hp_min;
-  const int lvl = in.level[ip];
-  int a = MS_KEEP;
-  if ((stretched || sheared) && !thin_skip && lvl > m.min_level)
-    a = MS_SPLIT;
-  else if (thick && in.hp[ip] > m.hp_min && lvl < m.max_level)
-    a = MS_CANDIDATE;
-  act[ip] = a;
+  act[ip] = ms_flag_one(m, dt, in, ip);
   partner[ip] = -1;
-}
-
-// Greedy pairing of one cell's candidates, ascending id; returns the number
-// of pairs formed.  `limit` stops pairing once *n_merge reaches it (serial
-// replay); the parallel kernel passes a null counter.
-__device__ int pair_cell(int b, int e, const int *cptcls, const MsInputs &in,
-                         double max_dv2, int *act, int *partner,
-                         int *n_merge, int limit) {
-  int formed = 0;
-  for (int i = b; i < e; ++i) {
-    if (n_merge && *n_merge >= limit) break;
-    const int q = cptcls[i];
-    if (act[q] != MS_CANDIDATE) continue;
-    double best = 1.7976931348623157e308;  // DBL_MAX
-    int bj = -1;
-    const double xq = in.x[q], yq = in.y[q], vxq = in.vpx[q], vyq = in.vpy[q];
-    for (int j = i + 1; j < e; ++j) {
-      const int q2 = cptcls[j];
-      if (act[q2] != MS_CANDIDATE) continue;
-      const double dx = xq - in.x[q2], dy = yq - in.y[q2];
-      const double d2 = dx * dx + dy * dy;
-      const double dvx = vxq - in.vpx[q2], dvy = vyq - in.vpy[q2];
-      if (d2 < best && (dvx * dvx + dvy * dvy) < max_dv2) {
-        best = d2;
-        bj = j;
-      }
-    }
-    if (bj >= 0) {
-      const int q2 = cptcls[bj];
-      act[q] = MS_PRIMARY;
-      act[q2] = MS_SECONDARY;
-      partner[q] = q2;
-      ++formed;
-      if (n_merge) ++*n_merge;
-    } else {
-      act[q] = MS_KEEP;
-    }
-  }
-  // candidates never paired (or left over when the cap stopped the loop)
-  for (int i = b; i < e; ++i)
-    if (act[cptcls[i]] == MS_CANDIDATE) act[cptcls[i]] = MS_KEEP;
-  return formed;
-}
-
-// Undo merges (first primary in id order) until the cell keeps min_ppc
-// particles; returns the number undone.
-__device__ int undo_cell(int b, int e, const int *cptcls, int min_ppc,
-                         int *act, int *partner) {
-  int projected = 0;
-  for (int i = b; i < e; ++i) {
-    const int a = act[cptcls[i]];
-    projected += a == MS_KEEP ? 1 : a == MS_SPLIT ? 2 : a == MS_PRIMARY ? 1 : 0;
-  }
-  int undone = 0;
-  while (projected < min_ppc) {
-    bool did = false;
-    for (int i = b; i < e; ++i) {
-      const int q = cptcls[i];
-      if (act[q] == MS_PRIMARY) {
-        const int q2 = partner[q];
-        if (q2 >= 0) act[q2] = MS_KEEP;
-        partner[q] = -1;
-        act[q] = MS_KEEP;
-        ++undone;
-        projected += 1;
-        did = true;
-        break;
-      }
-    }
-    if (!did) break;
-  }
-  return undone;
 }
 
 __global__ void ms_pair_kernel(int nc, MsParams m, MsInputs in,
@@ -137,23 +51,7 @@ __global__ void ms_serial_kernel(int nc, MsParams m, MsInputs in,
                                  const int *__restrict__ cptcls, int *act,
                                  int *partner, int *cnt) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int n_split = 0, n_merge = 0;
-  for (int cell = 0; cell < nc; ++cell) {
-    const int b = cstart[cell], e = cstart[cell + 1];
-    if (b == e) continue;
-    for (int i = b; i < e; ++i) {
-      const int q = cptcls[i];
-      if (act[q] == MS_SPLIT) {
-        if (n_split < m.max_ops) ++n_split;
-        else act[q] = MS_KEEP;
-      }
-    }
-    pair_cell(b, e, cptcls, in, m.max_dv2, act, partner, &n_merge, m.max_ops);
-    n_merge -= undo_cell(b, e, cptcls, m.min_ppc, act, partner);
-  }
-  cnt[0] = n_split;
-  cnt[1] = n_merge;
-  cnt[2] = n_merge;
+  ms_serial_cells(0, nc, m, in, cstart, cptcls, act, partner, cnt);
 }
 
 }  // namespace
@@ -222,8 +120,18 @@ extern "C" int amsc_ms_decide(const amsc_params *p, int np, double dt,
   if (!p || np < 0 || p->nrows <= 0 || p->ncols <= 0 || !counts)
     return fail(AMSC_ERR_INVALID, "amsc_ms_decide: bad arguments");
   const MsParams m = ms_params_from(*p);
-  MsInputs in = {d_x,      d_y,      d_Ap,  d_hp,  d_vpx_dx, d_vpx_dy,
-                 d_vpy_dx, d_vpy_dy, d_vpx, d_vpy, d_level};
+  MsInputs in;
+  in.x = d_x;
+  in.y = d_y;
+  in.hp = d_hp;
+  in.Ap = d_Ap;
+  in.vxdx = d_vpx_dx;
+  in.vxdy = d_vpx_dy;
+  in.vydx = d_vpy_dx;
+  in.vydy = d_vpy_dy;
+  in.vpx = d_vpx;
+  in.vpy = d_vpy;
+  in.level = d_level;
   int *cnt = nullptr;
   if (hipMalloc(&cnt, sizeof(int) * 4) != hipSuccess)
     return fail(AMSC_ERR_NOMEM, "amsc_ms_decide: counter allocation");

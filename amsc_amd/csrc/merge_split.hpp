@@ -1,4 +1,8 @@
-// merge_split.hpp -- device merge/split decisions (merge_split_ops_cmes.h).
+// merge_split.hpp -- device merge/split decisions (merge_split_ops_cmes.h):
+// the per-particle flag, the per-cell greedy pairing and the
+// min-particles-per-cell undo, shared by the standalone decision entry point
+// and the fused time loop.  Templated on the storage type R of the velocity,
+// gradient and area fields; the comparisons are evaluated in double.
 #pragma once
 
 #include "common.hpp"
@@ -19,10 +23,12 @@ struct MsParams {
   int min_level, max_level, min_ppc, max_ops;
 };
 
-struct MsInputs {
-  const double *x, *y, *Ap, *hp, *vxdx, *vxdy, *vydx, *vydy, *vpx, *vpy;
+template <typename R> struct MsIn {
+  const double *x, *y, *hp;
+  const R *Ap, *vxdx, *vxdy, *vydx, *vydy, *vpx, *vpy;
   const int *level;
 };
+using MsInputs = MsIn<double>;
 
 // ms_config from the solver parameters (gpu_offload.cpp:308-319).
 MsParams ms_params_from(const amsc_params &p);
@@ -34,6 +40,128 @@ int ms_decide(const MsParams &m, int np, int nc, const double *dt_dev,
               double dt_host, const MsInputs &in, const int *cstart,
               const int *cptcls, int *act, int *partner, int *cnt_dev,
               int *counts, hipStream_t st);
+
+// Action flag of one particle, merge_split_ops_cmes.h:143-172.
+template <typename R>
+__device__ __forceinline__ int ms_flag_one(const MsParams &m, double dt,
+                                           const MsIn<R> &in, int ip) {
+  const double r_i = sqrt((double)in.Ap[ip]);
+  const double exx = in.vxdx[ip], eyy = in.vydy[ip];
+  const double exy = 0.5 * ((double)in.vxdy[ip] + (double)in.vydx[ip]);
+  const double gdot = sqrt(2.0 * (exx * exx + eyy * eyy + 2.0 * exy * exy));
+  const bool stretched = r_i > m.r_split;
+  const bool sheared = gdot * dt > m.shear_split;
+  const bool thick = r_i < m.r_merge;
+  const bool thin_skip = in.hp[ip] < m.split_hp_min;
+  const int lvl = in.level[ip];
+  if ((stretched || sheared) && !thin_skip && lvl > m.min_level)
+    return MS_SPLIT;
+  if (thick && in.hp[ip] > m.hp_min && lvl < m.max_level) return MS_CANDIDATE;
+  return MS_KEEP;
+}
+
+// Greedy pairing of one cell's candidates, ascending id
+// (merge_split_ops_cmes.h:195-228); returns the number of pairs formed.
+// `limit` stops pairing once *n_merge reaches it (serial replay); the
+// parallel kernels pass a null counter.
+template <typename R>
+__device__ int pair_cell(int b, int e, const int *cptcls, const MsIn<R> &in,
+                         double max_dv2, int *act, int *partner, int *n_merge,
+                         int limit) {
+  int formed = 0;
+  for (int i = b; i < e; ++i) {
+    if (n_merge && *n_merge >= limit) break;
+    const int q = cptcls[i];
+    if (act[q] != MS_CANDIDATE) continue;
+    double best = 1.7976931348623157e308;  // DBL_MAX
+    int bj = -1;
+    const double xq = in.x[q], yq = in.y[q];
+    const double vxq = in.vpx[q], vyq = in.vpy[q];
+    for (int j = i + 1; j < e; ++j) {
+      const int q2 = cptcls[j];
+      if (act[q2] != MS_CANDIDATE) continue;
+      const double dx = xq - in.x[q2], dy = yq - in.y[q2];
+      const double d2 = dx * dx + dy * dy;
+      const double dvx = vxq - (double)in.vpx[q2];
+      const double dvy = vyq - (double)in.vpy[q2];
+      if (d2 < best && (dvx * dvx + dvy * dvy) < max_dv2) {
+        best = d2;
+        bj = j;
+      }
+    }
+    if (bj >= 0) {
+      const int q2 = cptcls[bj];
+      act[q] = MS_PRIMARY;
+      act[q2] = MS_SECONDARY;
+      partner[q] = q2;
+      ++formed;
+      if (n_merge) ++*n_merge;
+    } else {
+      act[q] = MS_KEEP;
+    }
+  }
+  // candidates never paired (or left over when the cap stopped the loop)
+  for (int i = b; i < e; ++i)
+    if (act[cptcls[i]] == MS_CANDIDATE) act[cptcls[i]] = MS_KEEP;
+  return formed;
+}
+
+// Undo merges (first primary in id order) until the cell keeps min_ppc
+// particles (merge_split_ops_cmes.h:230-249); returns the number undone.
+__device__ inline int undo_cell(int b, int e, const int *cptcls, int min_ppc,
+                                int *act, int *partner) {
+  int projected = 0;
+  for (int i = b; i < e; ++i) {
+    const int a = act[cptcls[i]];
+    projected += a == MS_KEEP ? 1 : a == MS_SPLIT ? 2 : a == MS_PRIMARY ? 1 : 0;
+  }
+  int undone = 0;
+  while (projected < min_ppc) {
+    bool did = false;
+    for (int i = b; i < e; ++i) {
+      const int q = cptcls[i];
+      if (act[q] == MS_PRIMARY) {
+        const int q2 = partner[q];
+        if (q2 >= 0) act[q2] = MS_KEEP;
+        partner[q] = -1;
+        act[q] = MS_KEEP;
+        ++undone;
+        projected += 1;
+        did = true;
+        break;
+      }
+    }
+    if (!did) break;
+  }
+  return undone;
+}
+
+// The serial decision loop over cells [c0, c1) with the global caps
+// (merge_split_ops_cmes.h:176-249), from flags already set; cnt[0..2] =
+// splits, merges, merges.
+template <typename R>
+__device__ void ms_serial_cells(int c0, int c1, const MsParams &m,
+                                const MsIn<R> &in, const int *cstart,
+                                const int *cptcls, int *act, int *partner,
+                                int *cnt) {
+  int n_split = 0, n_merge = 0;
+  for (int cell = c0; cell < c1; ++cell) {
+    const int b = cstart[cell], e = cstart[cell + 1];
+    if (b == e) continue;
+    for (int i = b; i < e; ++i) {
+      const int q = cptcls[i];
+      if (act[q] == MS_SPLIT) {
+        if (n_split < m.max_ops) ++n_split;
+        else act[q] = MS_KEEP;
+      }
+    }
+    pair_cell(b, e, cptcls, in, m.max_dv2, act, partner, &n_merge, m.max_ops);
+    n_merge -= undo_cell(b, e, cptcls, m.min_ppc, act, partner);
+  }
+  cnt[0] = n_split;
+  cnt[1] = n_merge;
+  cnt[2] = n_merge;
+}
 
 __device__ __forceinline__ void principal_stretch_dir(double exx, double eyy,
                                                       double exy, double *ux,

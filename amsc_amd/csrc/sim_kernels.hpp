@@ -1,0 +1,1142 @@
+// sim_kernels.hpp -- device kernels of the fused time loop (sim.hip), for
+// one scenario or a batch of scenarios, templated on the arithmetic type R
+// of the particle and nodal fields.
+//
+// Precision.  R = double is the plain FP64 path: every expression is the
+// oracle's, in the oracle's order, so results are bit-identical to the FP64
+// CPU restatement of optim_par.  R = float is the mixed path: positions,
+// depth hp, H, the well-balance terms (1 - H/hp and hp - H^2/hp, i.e. the
+// free-surface difference hp + Zp - eq_level), the bathymetry gradient, the
+// P2G mass and momentum sums and the conservation reductions are FP64;
+// velocities, stresses, friction, forces, velocity gradients, volumes,
+// masses and the nodal velocities are FP32, and so is their arithmetic.
+//
+// Batch layout.  Scenario s owns particles [poff[s], poff[s+1]) (device
+// offsets, moved by merge/split), cells [coff, coff + nrows*ncols) (the
+// binning key is the global cell coff + r + c*nrows), nodes
+// [noff, noff + (nrows+1)*(ncols+1)) and P2G tiles [toff, toff + ntiles).
+// Particle kernels find a particle's scenario from wscen (scenario of the
+// first particle of every 64-particle wavefront) and poff; P2G tiles from
+// tscen.
+#pragma once
+
+#include "binning.hpp"
+#include "common.hpp"
+#include "merge_split.hpp"
+
+namespace amsc {
+
+// ------------------------------------------------------------- layout
+enum PF : int {
+  // persistent state
+  PF_X, PF_Y, PF_VPX, PF_VPY, PF_HP, PF_VP, PF_MP, PF_H, PF_F11, PF_F12,
+  PF_F22, PF_FRX, PF_FRY,
+  // written on output / merge steps only
+  PF_AP, PF_FBX, PF_FBY, PF_VXDX, PF_VXDY, PF_VYDX, PF_VYDY, PF_APX, PF_APY,
+  PF_ZP, PF_HPZ, PF_DZX, PF_DZY, PF_FPX, PF_FPY, PF_MOMX, PF_MOMY,
+  PF_COUNT
+};
+constexpr int PF_CORE = PF_AP;
+
+// Fields stored in FP64 in both precisions; the others are of type R.
+__host__ __device__ constexpr bool pf_wide(int k) {
+  return k == PF_X || k == PF_Y || k == PF_HP || k == PF_H;
+}
+
+__host__ __device__ constexpr bool pf_extensive(int k) {
+  // merge_split_ops_cmes.h:49-50
+  return k == PF_MP || k == PF_VP || k == PF_AP || k == PF_MOMX ||
+         k == PF_MOMY;
+}
+
+enum NF : int {
+  NF_VVXL, NF_VVYL,  // always, type R
+  NF_AVX, NF_AVY, NF_MV, NF_MOMX, NF_MOMY, NF_FEXTX, NF_FEXTY, NF_FINTX,
+  NF_FINTY, NF_FRX, NF_FRY, NF_FPX, NF_FPY, NF_FTOTX, NF_FTOTY, NF_VVX,
+  NF_VVY,  // output steps only, FP64
+  NF_COUNT
+};
+__host__ __device__ constexpr bool nf_narrow(int k) {
+  return k == NF_VVXL || k == NF_VVYL;
+}
+
+struct PPtr {
+  void *f[PF_COUNT];
+  int *label, *level;
+};
+struct NPtr {
+  void *f[NF_COUNT];
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ T *fld(const PPtr &p, int k) {
+  return static_cast<T *>(p.f[k]);
+}
+
+// Device control block, one per scenario.
+struct Ctrl {
+  double t;        // time after the last step taken
+  double dt;       // dt of the last step taken
+  double t_start;  // time at the start of the last step taken
+  double cons[2];  // err_mass, err_mom of the last step
+  int it;          // steps taken
+  int active;      // 0 once the run has ended
+  int pad[2];
+  // max |vpx|, |vpy|, hp of the particles as bit patterns of non-negative
+  // doubles, by step parity, spread over 64 addresses so that the per-block
+  // atomics of the particle kernel do not all hit one line
+  unsigned long long cfl[2][64][3];
+};
+constexpr size_t kCtrlHead = 48;  // bytes before cfl (what the host reads)
+
+struct KParams {
+  int nrows, ncols;
+  double hx, hy, g, rho, mu, tauy, xi, tan_fa, bing, fric, cfl, eq_level,
+      dt_fixed, T;
+  int bc, nsteps, max_steps;
+};
+
+// One scenario of the batch.
+struct Scen {
+  KParams P;
+  Recip rc;            // FP64 basis-gradient factors
+  RecipT<float> rcf;   // the same, rounded to FP32
+  FastDiv fd;          // division by nrows
+  MsParams ms;
+  int ms_on, ms_interval;
+  int coff, noff;      // first global cell, node
+  int toff, ntiles, ntx;  // first P2G tile, tiles, tiles along x
+};
+
+template <typename R> __device__ __forceinline__ RecipT<R> recip_of(const Scen &S);
+template <>
+__device__ __forceinline__ RecipT<double> recip_of<double>(const Scen &S) {
+  return RecipT<double>{S.rc.ix, S.rc.nix, S.rc.iy, S.rc.niy};
+}
+template <>
+__device__ __forceinline__ RecipT<float> recip_of<float>(const Scen &S) {
+  return S.rcf;
+}
+
+struct Tables {
+  const Scen *scen;   // [S]
+  Ctrl *ctrl;         // [S]
+  const int *poff;    // [S+1]; poff[S] = particles in the batch
+  const int *wscen;   // [capacity/64]
+  int S;
+};
+
+__device__ __forceinline__ int scen_of(const Tables &T, int ip) {
+  int s = T.wscen[ip >> 6];
+  while (s + 1 < T.S && ip >= T.poff[s + 1]) ++s;
+  return s;
+}
+
+// Merge/split runs on step numbers divisible by ms_call_interval
+// (gpu_offload.cpp:321: it % interval == 0 after it++).
+__device__ __forceinline__ bool ms_due(const Scen &S, const Ctrl &C) {
+  return S.ms_on && S.ms_interval > 0 && C.active &&
+         C.it % S.ms_interval == 0;
+}
+
+// Which optional fields a step writes: none; what the next merge/split
+// reads (Ap, Fb, velocity gradients); every output field and the nodal set.
+enum : int { AUX_NONE = 0, AUX_MS = 1, AUX_FULL = 2 };
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(v, d);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------- prologue
+// One wavefront per scenario: end test, dt, t, it for the step about to run.
+__global__ void __launch_bounds__(64)
+    prologue_kernel(const Scen *__restrict__ scen, Ctrl *ctrls) {
+  const int lane = threadIdx.x;
+  const KParams &P = scen[blockIdx.x].P;
+  Ctrl *ctrl = &ctrls[blockIdx.x];
+  if (!ctrl->active) return;
+  const bool fixed = P.dt_fixed > 0.0;
+  const int it = ctrl->it;
+  // loop condition, optim_par.cpp:281; optional step cap for adaptive runs
+  const bool ended = (fixed ? !(it < P.nsteps) : !(ctrl->t < P.T)) ||
+                     (!fixed && P.max_steps > 0 && it >= P.max_steps);
+  if (ended) {
+    if (lane == 0) ctrl->active = 0;
+    return;
+  }
+  const int s = it + 1;  // this step
+  const unsigned long long m0 = wave_max_u64(ctrl->cfl[(s - 1) & 1][lane][0]);
+  const unsigned long long m1 = wave_max_u64(ctrl->cfl[(s - 1) & 1][lane][1]);
+  const unsigned long long m2 = wave_max_u64(ctrl->cfl[(s - 1) & 1][lane][2]);
+  double dt = ctrl->dt;
+  if (fixed) {
+    dt = P.dt_fixed;
+  } else if (it > 0) {
+    // optim_par.cpp:289-304: max |v| and max hp over the particles
+    const double mvx = __longlong_as_double((long long)m0);
+    const double mvy = __longlong_as_double((long long)m1);
+    const double hmax = __longlong_as_double((long long)m2);
+    const double a = sqrt(P.g * hmax) + mvx, b = sqrt(P.g * hmax) + mvy;
+    const double cel = fabs(a > b ? a : b);
+    const double minh = P.hx < P.hy ? P.hx : P.hy;
+    dt = P.cfl * minh / (1e-2 + cel);
+  }
+  ctrl->cfl[s & 1][lane][0] = 0ull;
+  ctrl->cfl[s & 1][lane][1] = 0ull;
+  ctrl->cfl[s & 1][lane][2] = 0ull;
+  if (lane == 0) {
+    ctrl->dt = dt;
+    ctrl->t_start = ctrl->t;
+    ctrl->t = ctrl->t + dt;
+    ctrl->it = s;
+  }
+}
+
+// ------------------------------------------------------------- step 0
+// optim_par.cpp:235-264 (cells from the initial binning): Zp, hpZ, H,
+// hydrostatic F, Fric = Ap*Fb with Fb = 0.
+template <typename R>
+__global__ void init_kernel(Tables T, PPtr p, const int *key,
+                            const double *__restrict__ Z,
+                            const double *mom_x_in, const double *mom_y_in) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= T.poff[T.S]) return;
+  const int sid = scen_of(T, ip);
+  const Scen &S = T.scen[sid];
+  const KParams &P = S.P;
+  const int cell = key[ip] - S.coff, r = cell % P.nrows, c = cell / P.nrows;
+  const Local L = local_coords(fld<double>(p, PF_X)[ip],
+                               fld<double>(p, PF_Y)[ip], c, r, P.hx, P.hy);
+  double zp = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    zp += shape(L, k) * Z[S.noff + node_id(k, c, r, P.nrows)];
+  const double hp = fld<double>(p, PF_HP)[ip];
+  fld<R>(p, PF_ZP)[ip] = (R)zp;
+  fld<R>(p, PF_HPZ)[ip] = (R)(hp + zp);
+  const double diff = P.eq_level - zp;
+  const double H = (P.eq_level > 0 && diff > 0) ? diff : 0.0;
+  fld<double>(p, PF_H)[ip] = H;
+  const double h_corr = hp > 1e-10 ? hp - (H * H) / hp : 0.0;
+  fld<R>(p, PF_F11)[ip] = (R)(-.5 * P.rho * P.g * h_corr);
+  fld<R>(p, PF_F12)[ip] = R(0);
+  fld<R>(p, PF_F22)[ip] = (R)(-.5 * P.rho * P.g * h_corr);
+  fld<R>(p, PF_FBX)[ip] = R(0);
+  fld<R>(p, PF_FBY)[ip] = R(0);
+  fld<R>(p, PF_FRX)[ip] = fld<R>(p, PF_AP)[ip] * R(0);
+  fld<R>(p, PF_FRY)[ip] = fld<R>(p, PF_AP)[ip] * R(0);
+  const int zero_fields[] = {PF_VXDX, PF_VXDY, PF_VYDX, PF_VYDY, PF_APX,
+                             PF_APY,  PF_DZX,  PF_DZY,  PF_FPX,  PF_FPY};
+#pragma unroll
+  for (int k = 0; k < 10; ++k) fld<R>(p, zero_fields[k])[ip] = R(0);
+  fld<R>(p, PF_MOMX)[ip] = (R)mom_x_in[ip];
+  fld<R>(p, PF_MOMY)[ip] = (R)mom_y_in[ip];
+  p.label[ip] = ip - T.poff[sid];
+  p.level[ip] = 0;
+}
+
+// Cell key and arrival slot of every particle (re-binning after creation or
+// merge/split).
+__global__ void key_kernel(Tables T, const double *__restrict__ x,
+                           const double *__restrict__ y, int *key, int *slot,
+                           int *count) {
+  const int np = T.poff[T.S];
+  const int b0 = blockIdx.x * blockDim.x;
+  if (b0 >= np) return;
+  const int ip = b0 + threadIdx.x;
+  const bool valid = ip < np;
+  int k = 0;
+  if (valid) {
+    const Scen &S = T.scen[scen_of(T, ip)];
+    k = S.coff + cell_of(x[ip], y[ip], S.P.nrows, S.P.ncols, S.P.hx, S.P.hy);
+  }
+  const int sl = wave_slot(count, k, valid);
+  if (valid) {
+    key[ip] = k;
+    slot[ip] = sl;
+  }
+}
+
+// ------------------------------------------------------------- P2G
+// Output-only destinations of p2g_nodal (read from device memory only on
+// steps that write them, which keeps the kernel's argument block small).
+struct P2GOut {
+  void *dzx, *dzy, *fpx, *fpy, *momx, *momy;  // particle fields, type R
+  double *nod[NF_COUNT];                       // nodal set (FP64 ones)
+};
+
+struct P2GArgs {
+  Tables T;
+  const int *tscen;
+  int aux, mom_stored;
+  const int *cstart, *cptcls;
+  const double *x, *y, *hp, *H;
+  // type R: Mp, momentum source (vpx/vpy, or the DATA momentum on the first
+  // step), Vp, friction force, stress
+  const void *Mp, *m1, *m2, *Vp, *frx, *fry, *f11, *f12, *f22;
+  const double *Z;
+  void *vvxL, *vvyL;  // type R
+  double *partials;   // [tiles][6] or null
+  const P2GOut *out;
+};
+
+template <typename R> struct P2GRaw {
+  double x, y, hp, H;
+  R Mp, m1, m2, Vp, frx, fry, f11, f12, f22;
+};
+// Where a tile-list position lives: particle id, cell column segment and
+// cell row within the segment.
+struct P2GIdx {
+  int ip, s, j;
+};
+
+template <typename R>
+__device__ __forceinline__ void p2g_fields(const P2GArgs &a, int ip,
+                                           P2GRaw<R> &W) {
+  W.x = a.x[ip];
+  W.y = a.y[ip];
+  W.Mp = static_cast<const R *>(a.Mp)[ip];
+  W.m1 = static_cast<const R *>(a.m1)[ip];
+  W.m2 = static_cast<const R *>(a.m2)[ip];
+  W.Vp = static_cast<const R *>(a.Vp)[ip];
+  W.hp = a.hp[ip];
+  W.H = a.H[ip];
+  W.frx = static_cast<const R *>(a.frx)[ip];
+  W.fry = static_cast<const R *>(a.fry)[ip];
+  W.f11 = static_cast<const R *>(a.f11)[ip];
+  W.f12 = static_cast<const R *>(a.f12)[ip];
+  W.f22 = static_cast<const R *>(a.f22)[ip];
+}
+
+// Staged particle i lives at i + i/16 in each LDS array: the node threads of
+// a wavefront walk neighbouring cells in lockstep, i.e. at a stride of about
+// 4 particles, which without the pad maps 16 lanes onto one bank.
+__device__ __forceinline__ int lds_slot(int i) { return i + (i >> 4); }
+
+// P2G + nodal update for one tile of TC x TR nodes.
+//
+// The tile's particles are the CSR ranges of its (TC+1) x (TR+1) cells,
+// taken column segment by column segment ("tile list"), and staged through
+// LDS TC*TR at a time.  Each chunk goes through a two-stage pipeline: while
+// chunk k is folded, the fields of chunk k+1 and the particle ids of chunk
+// k+2 are in flight.  A per-chunk owner map (cell of every tile-list
+// position, built cell-parallel) turns a position into its particle id with
+// one LDS read.  Each node thread folds its four cells in ascending cell id,
+// each cell in ascending particle id: the serial p2g order, so the sums are
+// the oracle's bit for bit.  Workgroups are mapped to tiles so that each of
+// the 8 XCDs (workgroup i runs on XCD i mod 8) gets a contiguous range of
+// tiles, whose shared halo cells then hit the same L2.
+template <typename R, int TC, int TR>
+__global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
+  constexpr int NT = TC * TR, CH = NT, CHP = CH + CH / 16, SEG = TC + 1;
+  static_assert(TR + 1 < 256 && SEG < 256, "owner map packs (s, j) in 16 bits");
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int xcd = b & 7, q8 = nb >> 3, rem = nb & 7;
+  const int tile = xcd * q8 + (xcd < rem ? xcd : rem) + (b >> 3);
+  const int sid = a.tscen[tile];
+  const Scen &sc = a.T.scen[sid];
+  Ctrl *ctrl = &a.T.ctrl[sid];
+  if (!ctrl->active) return;
+  const int nrows = sc.P.nrows, ncols = sc.P.ncols;
+  const double hx = sc.P.hx, hy = sc.P.hy;
+  const Recip rc = sc.rc;
+  const RecipT<R> rcr = recip_of<R>(sc);
+  const int lt = tile - sc.toff;
+  const int c0 = (lt % sc.ntx) * TC, r0 = (lt / sc.ntx) * TR;
+  const int t = threadIdx.x;
+  const int lr = t % TR, lc = t / TR;
+  const int c = c0 + lc, r = r0 + lr;
+  const bool node_ok = c <= ncols && r <= nrows;
+  const int *cstart = a.cstart + sc.coff;
+
+  __shared__ int s_cs[SEG][TR + 2];  // cstart, cell rows rl..rh+1
+  __shared__ int s_sego[SEG + 1];    // tile-list offset per column
+  __shared__ double s_Z[(TC + 2) * (TR + 2)];
+  __shared__ double s_w[4][CHP];     // xi, eta, mom x, mom y (FP64)
+  __shared__ R s_n[9][CHP];          // Mp, Fpx, Fpy, frx, fry, F11, F12, F22, Vp
+  __shared__ unsigned short s_own[CH];
+  __shared__ double s_red[6][NT / 64];
+
+  const int ccl = c0 - 1 < 0 ? 0 : c0 - 1;
+  const int cch = c0 + TC - 1 > ncols - 1 ? ncols - 1 : c0 + TC - 1;
+  const int rl = r0 - 1 < 0 ? 0 : r0 - 1;
+  const int rh = r0 + TR - 1 > nrows - 1 ? nrows - 1 : r0 + TR - 1;
+  const int nseg = cch - ccl + 1, nr = rh - rl + 1;
+
+  for (int i = t; i < nseg * (nr + 1); i += NT) {
+    const int s = i / (nr + 1), j = i % (nr + 1);
+    s_cs[s][j] = cstart[(ccl + s) * nrows + rl + j];
+  }
+  __syncthreads();
+  if (t == 0) {
+    int o = 0;
+    for (int s = 0; s < nseg; ++s) {
+      s_sego[s] = o;
+      o += s_cs[s][nr] - s_cs[s][0];
+    }
+    s_sego[nseg] = o;
+  }
+  __syncthreads();
+  const int L = s_sego[nseg];
+
+  // owner map of the chunk starting at base (positions base .. base+CH)
+  auto build_own = [&](int base) {
+    if (base >= L) return;
+    for (int ci = t; ci < nseg * nr; ci += NT) {
+      const int s = ci / nr, j = ci % nr;
+      const int lo0 = s_sego[s] + (s_cs[s][j] - s_cs[s][0]);
+      const int hi0 = lo0 + (s_cs[s][j + 1] - s_cs[s][j]);
+      const int lo = lo0 > base ? lo0 : base;
+      const int hi = hi0 < base + CH ? hi0 : base + CH;
+      for (int q = lo; q < hi; ++q)
+        s_own[q - base] = (unsigned short)((s << 8) | j);
+    }
+  };
+  auto fetch_idx = [&](int q, P2GIdx &I) {
+    const int v = s_own[q % CH];
+    I.s = v >> 8;
+    I.j = v & 0xFF;
+    I.ip = a.cptcls[s_cs[I.s][0] + (q - s_sego[I.s])];
+  };
+
+  double pm = 0, ppx = 0, ppy = 0;  // particle sums (owned cells)
+  double Mv = 0, mvx = 0, mvy = 0;
+  R FPx = 0, FPy = 0, Frx = 0, Fry = 0, Fix = 0, Fiy = 0;
+
+  if (L > 0) {
+    P2GIdx I1, I2, I3;
+    P2GRaw<R> W;
+    build_own(0);
+    // Z of nodes (c0-1 .. c0+TC, r0-1 .. r0+TR)
+    for (int i = t; i < (TC + 2) * (TR + 2); i += NT) {
+      const int zc = c0 - 1 + i / (TR + 2), zr = r0 - 1 + i % (TR + 2);
+      s_Z[i] = (zc >= 0 && zc <= ncols && zr >= 0 && zr <= nrows)
+                   ? a.Z[sc.noff + zr + zc * (nrows + 1)]
+                   : 0.0;
+    }
+    // this node's four cells as tile-list ranges, ascending cell id:
+    // (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r) corner 0
+    int tb[4], te[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cc = c - 1 + (k >> 1), cr = r - 1 + (k & 1);
+      tb[k] = te[k] = 0;
+      if (node_ok && cc >= 0 && cc < ncols && cr >= 0 && cr < nrows) {
+        const int s = cc - ccl, j = cr - rl;
+        tb[k] = s_sego[s] + (s_cs[s][j] - s_cs[s][0]);
+        te[k] = s_sego[s] + (s_cs[s][j + 1] - s_cs[s][0]);
+      }
+    }
+    __syncthreads();
+    if (t < L) fetch_idx(t, I1);
+    __syncthreads();
+    build_own(CH);
+    __syncthreads();
+    if (CH + t < L) fetch_idx(CH + t, I2);
+    if (t < L) p2g_fields<R>(a, I1.ip, W);
+    __syncthreads();  // s_own is read above and rebuilt in the loop
+
+    for (int base = 0; base < L; base += CH) {
+      if (base + t < L) {
+        // the particle's cell is the CSR cell it was found in
+        const int pc = ccl + I1.s, pr = rl + I1.j;
+        const Local Lc = local_coords(W.x, W.y, pc, pr, hx, hy);
+        // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
+        const int zb = (pc - (c0 - 1)) * (TR + 2) + (pr - (r0 - 1));
+        const double z0 = s_Z[zb], z1 = s_Z[zb + 1];
+        const double z2 = s_Z[zb + (TR + 2)], z3 = s_Z[zb + (TR + 3)];
+        double dzx = 0.0, dzy = 0.0;
+        dzx += shape_dx(Lc, 0, rc) * z0;
+        dzy += shape_dy(Lc, 0, rc) * z0;
+        dzx += shape_dx(Lc, 1, rc) * z1;
+        dzy += shape_dy(Lc, 1, rc) * z1;
+        dzx += shape_dx(Lc, 2, rc) * z2;
+        dzy += shape_dy(Lc, 2, rc) * z2;
+        dzx += shape_dx(Lc, 3, rc) * z3;
+        dzy += shape_dy(Lc, 3, rc) * z3;
+        // gravity along the slope with the well-balance factor,
+        // optim_par.cpp:450-453
+        const double wb = W.hp > 1e-10 ? 1.0 - (W.H / W.hp) : 1.0;
+        const double Vpd = W.Vp;
+        const double Fpx = -sc.P.g * Vpd * sc.P.rho * dzx * wb;
+        const double Fpy = -sc.P.g * Vpd * sc.P.rho * dzy * wb;
+        // mom_p = vpx * Mp (optim_par.cpp:600-601), DATA momentum at step 1
+        const double Mpd = W.Mp;
+        const double mx = a.mom_stored ? (double)W.m1 : (double)W.m1 * Mpd;
+        const double my = a.mom_stored ? (double)W.m2 : (double)W.m2 * Mpd;
+        const int ts = lds_slot(t);
+        s_w[0][ts] = Lc.xi;
+        s_w[1][ts] = Lc.eta;
+        s_w[2][ts] = mx;
+        s_w[3][ts] = my;
+        s_n[0][ts] = W.Mp;
+        s_n[1][ts] = (R)Fpx;
+        s_n[2][ts] = (R)Fpy;
+        s_n[3][ts] = W.frx;
+        s_n[4][ts] = W.fry;
+        s_n[5][ts] = W.f11;
+        s_n[6][ts] = W.f12;
+        s_n[7][ts] = W.f22;
+        s_n[8][ts] = W.Vp;
+        const bool owned = pc >= c0 && pc < c0 + TC && pr >= r0 && pr < r0 + TR;
+        if (owned) {
+          pm += Mpd;
+          ppx += mx;
+          ppy += my;
+          if (a.aux >= AUX_FULL) {
+            const P2GOut *o = a.out;
+            static_cast<R *>(o->dzx)[I1.ip] = (R)dzx;
+            static_cast<R *>(o->dzy)[I1.ip] = (R)dzy;
+            static_cast<R *>(o->fpx)[I1.ip] = (R)Fpx;
+            static_cast<R *>(o->fpy)[I1.ip] = (R)Fpy;
+            static_cast<R *>(o->momx)[I1.ip] = (R)mx;
+            static_cast<R *>(o->momy)[I1.ip] = (R)my;
+          }
+        }
+      }
+      build_own(base + 2 * CH);
+      __syncthreads();
+      // in flight during the fold: ids of chunk k+2, fields of chunk k+1
+      if (base + 2 * CH + t < L) fetch_idx(base + 2 * CH + t, I3);
+      if (base + CH + t < L) p2g_fields<R>(a, I2.ip, W);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int lo = tb[k] > base ? tb[k] : base;
+        const int hi = te[k] < base + CH ? te[k] : base + CH;
+        const int corner = 3 - k;
+        for (int qq = lo; qq < hi; ++qq) {
+          const int i = lds_slot(qq - base);
+          const double xi = s_w[0][i], eta = s_w[1][i];
+          const double N = shape_t<double>(xi, eta, corner);
+          Mv += N * (double)s_n[0][i];
+          mvx += N * s_w[2][i];
+          mvy += N * s_w[3][i];
+          const R Nr = (R)N;
+          FPx += Nr * s_n[1][i];
+          FPy += Nr * s_n[2][i];
+          Frx += Nr * s_n[3][i];
+          Fry += Nr * s_n[4][i];
+          const R xr = (R)xi, er = (R)eta;
+          const R Nx = shape_dx_t<R>(xr, er, corner, rcr);
+          const R Ny = shape_dy_t<R>(xr, er, corner, rcr);
+          const R vp = s_n[8][i];
+          Fix += (Nx * s_n[5][i] + Ny * s_n[6][i]) * vp;
+          Fiy += (Nx * s_n[6][i] + Ny * s_n[7][i]) * vp;
+        }
+      }
+      __syncthreads();
+      I1 = I2;
+      I2 = I3;
+    }
+  }
+
+  if (node_ok && (L > 0 || a.aux >= AUX_FULL)) {
+    const int n = sc.noff + r + c * (nrows + 1);
+    const double dt = ctrl->dt;
+    // force balance and nodal velocities, optim_par.cpp:467-497 (D1)
+    const R Fex = FPx + Frx, Fey = FPy + Fry;
+    const R Ftx = Fex - Fix, Fty = Fey - Fiy;
+    double mx = dt * (double)Ftx + mvx, my = dt * (double)Fty + mvy;
+    double ax = Mv > 1.e-2 ? (double)Ftx / Mv : 0.0;
+    double ay = Mv > 1.e-2 ? (double)Fty / Mv : 0.0;
+    double vx = Mv > 1.e-2 ? mx / Mv : 0.0;
+    double vy = Mv > 1.e-2 ? my / Mv : 0.0;
+    double vxL = dt * ax + vx, vyL = dt * ay + vy;
+    // boundary conditions, optim_par.cpp:503-529 (nodes of the first/last
+    // cell column: x components; first/last cell row: y components)
+    if (sc.P.bc) {
+      if (c == 0 || c == 1 || c == ncols - 1 || c == ncols) {
+        ax = 0.0; vx = 0.0; vxL = 0.0; mx = 0.0;
+      }
+      if (r == 0 || r == 1 || r == nrows - 1 || r == nrows) {
+        ay = 0.0; vy = 0.0; vyL = 0.0; my = 0.0;
+      }
+    }
+    static_cast<R *>(a.vvxL)[n] = (R)vxL;
+    static_cast<R *>(a.vvyL)[n] = (R)vyL;
+    if (a.aux >= AUX_FULL) {
+      double *const *nod = a.out->nod;
+      nod[NF_AVX][n] = ax;
+      nod[NF_AVY][n] = ay;
+      nod[NF_MV][n] = Mv;
+      nod[NF_MOMX][n] = mx;
+      nod[NF_MOMY][n] = my;
+      nod[NF_FEXTX][n] = Fex;
+      nod[NF_FEXTY][n] = Fey;
+      nod[NF_FINTX][n] = Fix;
+      nod[NF_FINTY][n] = Fiy;
+      nod[NF_FRX][n] = Frx;
+      nod[NF_FRY][n] = Fry;
+      nod[NF_FPX][n] = FPx;
+      nod[NF_FPY][n] = FPy;
+      nod[NF_FTOTX][n] = Ftx;
+      nod[NF_FTOTY][n] = Fty;
+      nod[NF_VVX][n] = vx;
+      nod[NF_VVY][n] = vy;
+    }
+  }
+
+  if (a.partials) {
+    // conservation sums (optim_par.cpp:391-428), FP64, fixed order: tile
+    // partials here, the scenario's reduction in conservation_kernel
+    const int lane = t & 63, w = t >> 6;
+    const double v[6] = {pm, ppx, ppy, node_ok ? Mv : 0.0, node_ok ? mvx : 0.0,
+                         node_ok ? mvy : 0.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double sm = wave_sum(v[k]);
+      if (lane == 0) s_red[k][w] = sm;
+    }
+    __syncthreads();
+    if (t < 6) {
+      double sm = 0.0;
+      for (int ww = 0; ww < NT / 64; ++ww) sm += s_red[t][ww];
+      a.partials[tile * 6 + t] = sm;
+    }
+  }
+}
+
+// err_mass, err_mom of every scenario from its tiles' partials, one
+// workgroup per scenario, optim_par.cpp:416-425.
+__global__ void __launch_bounds__(256)
+    conservation_kernel(const Scen *__restrict__ scen,
+                        const double *__restrict__ partials, Ctrl *ctrls) {
+  const Scen &sc = scen[blockIdx.x];
+  Ctrl *ctrl = &ctrls[blockIdx.x];
+  if (!ctrl->active) return;
+  __shared__ double s[6][4];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < sc.ntiles; i += 256)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] += partials[(sc.toff + i) * 6 + k];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double sm = wave_sum(v[k]);
+    if (lane == 0) s[k][w] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[6];
+    for (int k = 0; k < 6; ++k) tot[k] = ((s[k][0] + s[k][1]) + s[k][2]) + s[k][3];
+    const double tm = tot[0], tx = tot[1], ty = tot[2];
+    const double nm = tot[3], nx = tot[4], ny = tot[5];
+    const double am = fabs(tx) > fabs(ty) ? fabs(tx) : fabs(ty);
+    const double norm = 1.0 > am ? 1.0 : am;
+    const double ex = fabs(tx - nx) / norm, ey = fabs(ty - ny) / norm;
+    ctrl->cons[0] = fabs(tm - nm) / tm;
+    ctrl->cons[1] = ex > ey ? ex : ey;
+  }
+}
+
+// ------------------------------------------------------------- particles
+struct PartArgs {
+  Tables T;
+  PPtr p;
+  const int *key_in;  // step-start cell of each particle
+  int *key_out, *slot, *count;
+  const void *vvxL, *vvyL;  // type R
+  const double *avx, *avy;
+  const double *Z;
+};
+
+// G2P, PIC velocity, advection, G2PD at the new position with the
+// step-start cell, height update, Voellmy, Bingham, friction force and the
+// new cell key of one particle of an active scenario.
+template <typename R, int AUX>
+__device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
+                                              double dt, int ip, int &newkey,
+                                              double &vmx, double &vmy,
+                                              double &hmx) {
+  const KParams &P = sc.P;
+  const PPtr &p = a.p;
+  const int nrows = P.nrows;
+  const double hx = P.hx, hy = P.hy;
+  const int cell = a.key_in[ip] - sc.coff, c = fdiv(cell, sc.fd),
+            r = cell - c * nrows;
+  int nid[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) nid[k] = sc.noff + node_id(k, c, r, nrows);
+  R gx[4], gy[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    gx[k] = static_cast<const R *>(a.vvxL)[nid[k]];
+    gy[k] = static_cast<const R *>(a.vvyL)[nid[k]];
+  }
+  const double x = fld<double>(p, PF_X)[ip], y = fld<double>(p, PF_Y)[ip];
+  const Local L0 = local_coords(x, y, c, r, hx, hy);
+  // g2p(vvxL, vvyL), optim_par.cpp:542
+  const R xi0 = (R)L0.xi, eta0 = (R)L0.eta;
+  R vxL = 0, vyL = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const R N = shape_t<R>(xi0, eta0, k);
+    vxL += N * gx[k];
+    vyL += N * gy[k];
+  }
+  // PIC velocity, advection and clamp, optim_par.cpp:548-564
+  const R vpx = vxL, vpy = vyL;
+  double xn = x + dt * (double)vpx, yn = y + dt * (double)vpy;
+  const double eps = 1e-10, Lx = P.ncols * P.hx, Ly = P.nrows * P.hy;
+  if (xn < eps) xn = eps;
+  if (xn > Lx - eps) xn = Lx - eps;
+  if (yn < eps) yn = eps;
+  if (yn > Ly - eps) yn = Ly - eps;
+  // g2pd(vvxL, vvyL) at the new position, step-start cell (D18),
+  // optim_par.cpp:578
+  const Local L1 = local_coords(xn, yn, c, r, hx, hy);
+  const R xi1 = (R)L1.xi, eta1 = (R)L1.eta;
+  const RecipT<R> rcr = recip_of<R>(sc);
+  R vxdx = 0, vxdy = 0, vydx = 0, vydy = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const R Nx = shape_dx_t<R>(xi1, eta1, k, rcr);
+    const R Ny = shape_dy_t<R>(xi1, eta1, k, rcr);
+    vxdx += Nx * gx[k];
+    vxdy += Ny * gx[k];
+    vydx += Nx * gy[k];
+    vydy += Ny * gy[k];
+  }
+  // height update with floor, area, optim_par.cpp:595-607 (FP64)
+  const R div = vxdx + vydy;
+  double h = fld<double>(p, PF_HP)[ip] / (1 + dt * (double)div);
+  h = h < 1e-2 ? 1e-2 : h;
+  const R Vp = fld<R>(p, PF_VP)[ip];
+  const R hR = (R)h;
+  const R Ap = h > 1.e-2 ? Vp / hR : R(0);
+  const R rho = (R)P.rho, g = (R)P.g;
+  // Voellmy basal friction, optim_par.cpp:613-626
+  R Fbx = 0, Fby = 0;
+  {
+    const R nv = sqrt(vpx * vpx + vpy * vpy);
+    if (P.fric > 0 && nv > R(1e-10) && P.xi > 0) {
+      const R v2 = vpx * vpx + vpy * vpy;
+      const R fric = (R)P.fric, tan_fa = (R)P.tan_fa, xi = (R)P.xi;
+      Fbx = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpx / nv;
+      Fby = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpy / nv;
+    }
+  }
+  // Bingham stress, stress_tensor_t::operator(), optim_par.cpp:68-120
+  R F11, F12, F22;
+  {
+    const R A = R(3) / R(2), C = R(65) / R(32);
+    const R mu = (R)P.mu, tau_Y = (R)P.tauy;
+    const R nrm = sqrt(vpx * vpx + vpy * vpy);
+    const R ALF = hR > R(1.e-3) ? (R(6) * mu * nrm) / ((hR + R(0.001)) * tau_Y) : R(0);
+    const R B = R(-114) / R(32) - ALF;
+    // Z1 = (-B + sq)/(2A) is selected only if it lies in [0, 1]; when
+    // -B + sq > 3.0000001 it cannot (2A = 3), so Z2 is taken without
+    // evaluating Z1.  Same values as computing both.
+    const R sq = sqrt(B * B - R(4) * A * C);
+    const R num1 = -B + sq;
+    R ZZ;
+    if (num1 > R(3.0000001)) {
+      ZZ = (-B - sq) / (R(2) * A);
+    } else {
+      const R Z1 = num1 / (R(2) * A);
+      ZZ = fabs(Z1 - R(.5)) <= R(.5) ? Z1 : (-B - sq) / (R(2) * A);
+    }
+    const R D_xx = vxdx;
+    const R D_xy = R(0.5) * (vxdy + vydx);
+    const R D_yy = vydy;
+    const R D_zx = hR > R(1.e-3) ? R(0.5) * (R(3) / (R(2) + ZZ)) * (vpx / (hR + R(0.001))) : R(0);
+    const R D_zy = hR > R(1.e-3) ? R(0.5) * (R(3) / (R(2) + ZZ)) * (vpy / (hR + R(0.001))) : R(0);
+    const R D_zz = -(vxdx + vydy);
+    const R invII = R(0.5) * (D_xx * D_xx + D_yy * D_yy + D_zz * D_zz) +
+                    D_zx * D_zx + D_zy * D_zy + D_xy * D_xy;
+    const R sig_xx = invII != R(0) ? (tau_Y / sqrt(invII) + R(2) * mu) * D_xx : R(0);
+    const R sig_xy = invII != R(0) ? (tau_Y / sqrt(invII) + R(2) * mu) * D_xy : R(0);
+    const R sig_yy = invII != R(0) ? (tau_Y / sqrt(invII) + R(2) * mu) * D_yy : R(0);
+    // hydrostatic part with the well-balance correction, FP64
+    const double H = fld<double>(p, PF_H)[ip];
+    const double h_corr = h > 1e-10 ? h - (H * H) / h : 0.0;
+    if (sizeof(R) == sizeof(double)) {
+      F11 = P.bing * sig_xx - .5 * P.rho * P.g * h_corr;
+      F12 = P.bing * sig_xy;
+      F22 = P.bing * sig_yy - .5 * P.rho * P.g * h_corr;
+    } else {
+      const R pres = (R)(.5 * P.rho * P.g * h_corr);
+      const R bing = (R)P.bing;
+      F11 = bing * sig_xx - pres;
+      F12 = bing * sig_xy;
+      F22 = bing * sig_yy - pres;
+    }
+  }
+  fld<double>(p, PF_X)[ip] = xn;
+  fld<double>(p, PF_Y)[ip] = yn;
+  fld<R>(p, PF_VPX)[ip] = vpx;
+  fld<R>(p, PF_VPY)[ip] = vpy;
+  fld<double>(p, PF_HP)[ip] = h;
+  fld<R>(p, PF_F11)[ip] = F11;
+  fld<R>(p, PF_F12)[ip] = F12;
+  fld<R>(p, PF_F22)[ip] = F22;
+  // friction force for the next P2G, optim_par.cpp:456-459
+  fld<R>(p, PF_FRX)[ip] = Ap * Fbx;
+  fld<R>(p, PF_FRY)[ip] = Ap * Fby;
+  if (AUX >= AUX_MS) {
+    fld<R>(p, PF_AP)[ip] = Ap;
+    fld<R>(p, PF_FBX)[ip] = Fbx;
+    fld<R>(p, PF_FBY)[ip] = Fby;
+    fld<R>(p, PF_VXDX)[ip] = vxdx;
+    fld<R>(p, PF_VXDY)[ip] = vxdy;
+    fld<R>(p, PF_VYDX)[ip] = vydx;
+    fld<R>(p, PF_VYDY)[ip] = vydy;
+  }
+  if (AUX >= AUX_FULL) {
+    double ax = 0.0, ay = 0.0, zp = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double N = shape(L0, k);
+      ax += N * a.avx[nid[k]];
+      ay += N * a.avy[nid[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) zp += shape(L1, k) * a.Z[nid[k]];
+    fld<R>(p, PF_APX)[ip] = (R)ax;
+    fld<R>(p, PF_APY)[ip] = (R)ay;
+    fld<R>(p, PF_ZP)[ip] = (R)zp;
+    fld<R>(p, PF_HPZ)[ip] = (R)(h + zp);
+    const R Mp = fld<R>(p, PF_MP)[ip];
+    fld<R>(p, PF_MOMX)[ip] = vpx * Mp;
+    fld<R>(p, PF_MOMY)[ip] = vpy * Mp;
+  }
+  newkey = sc.coff + cell_of(xn, yn, P.nrows, P.ncols, hx, hy);
+  vmx = fabs((double)vpx);
+  vmy = fabs((double)vpy);
+  hmx = h > 0.0 ? h : 0.0;
+}
+
+// One thread per particle.  Particles of a scenario whose run has ended
+// keep their cell (and are binned again).  The main launch (EDGE = false)
+// takes the workgroups whose particles all belong to one scenario: the
+// scenario's parameters are then workgroup-uniform, and the CFL maxima are
+// reduced in the workgroup and added with one atomic per value into one of
+// 64 slots.  The few workgroups that straddle a scenario boundary are left
+// to a second launch (EDGE = true, one workgroup per boundary) that looks
+// the scenario up per particle and adds the maxima per particle.
+template <typename R, int AUX, int BS, bool EDGE>
+__global__ void __launch_bounds__(BS)
+    __attribute__((amdgpu_waves_per_eu(EDGE ? 1 : AUX == AUX_FULL ? 4 : 6)))
+    particle_kernel(PartArgs a) {
+  const Tables &T = a.T;
+  const int np = T.poff[T.S];
+  int blk = blockIdx.x;
+  if (EDGE) {
+    // boundary between scenarios j and j+1; the first boundary inside a
+    // workgroup's range owns that workgroup
+    const int j = blockIdx.x;
+    if (j + 1 >= T.S) return;
+    const int pb = T.poff[j + 1];
+    if (pb >= np) return;
+    blk = pb / BS;
+    if (j > 0 && T.poff[j] / BS == blk) return;
+  }
+  const int b0 = blk * BS;
+  if (b0 >= np) return;
+  int sb = T.wscen[b0 >> 6];
+  while (sb + 1 < T.S && b0 >= T.poff[sb + 1]) ++sb;
+  const int last = (b0 + BS < np ? b0 + BS : np) - 1;
+  const bool uniform = last < T.poff[sb + 1];
+  if (uniform == EDGE) return;
+  const int ip = b0 + threadIdx.x;
+  const bool valid = ip < np;
+  int sid = sb;
+  if (EDGE && valid)
+    while (sid + 1 < T.S && ip >= T.poff[sid + 1]) ++sid;
+  const Ctrl &C = T.ctrl[sid];
+  const bool act = C.active != 0;
+  int newkey = 0;
+  double vmx = 0.0, vmy = 0.0, hmx = 0.0;
+  if (valid) {
+    if (act)
+      particle_body<R, AUX>(a, T.scen[sid], C.dt, ip, newkey, vmx, vmy, hmx);
+    else
+      newkey = a.key_in[ip];
+  }
+  const int sl = wave_slot(a.count, newkey, valid);
+  if (valid) {
+    a.key_out[ip] = newkey;
+    a.slot[ip] = sl;
+  }
+  if (EDGE) {
+    if (valid && act) {
+      unsigned long long *m = T.ctrl[sid].cfl[C.it & 1][ip & 63];
+      atomicMax(&m[0], (unsigned long long)__double_as_longlong(vmx));
+      atomicMax(&m[1], (unsigned long long)__double_as_longlong(vmy));
+      atomicMax(&m[2], (unsigned long long)__double_as_longlong(hmx));
+    }
+    return;
+  }
+  if (!act) return;
+  __shared__ double s_cfl[3][BS / 64];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    vmx = fmax(vmx, __shfl_xor(vmx, d));
+    vmy = fmax(vmy, __shfl_xor(vmy, d));
+    hmx = fmax(hmx, __shfl_xor(hmx, d));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_cfl[0][w] = vmx;
+    s_cfl[1][w] = vmy;
+    s_cfl[2][w] = hmx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BS / 64; ++k) {
+      vmx = fmax(vmx, s_cfl[0][k]);
+      vmy = fmax(vmy, s_cfl[1][k]);
+      hmx = fmax(hmx, s_cfl[2][k]);
+    }
+    unsigned long long *m = T.ctrl[sb].cfl[C.it & 1][blk & 63];
+    atomicMax(&m[0], (unsigned long long)__double_as_longlong(vmx));
+    atomicMax(&m[1], (unsigned long long)__double_as_longlong(vmy));
+    atomicMax(&m[2], (unsigned long long)__double_as_longlong(hmx));
+  }
+}
+
+// ------------------------------------------------------------- merge/split
+template <typename R>
+__global__ void ens_flag_kernel(Tables T, MsIn<R> in, int *act, int *partner) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= T.poff[T.S]) return;
+  const int sid = scen_of(T, ip);
+  const Scen &sc = T.scen[sid];
+  const Ctrl &C = T.ctrl[sid];
+  act[ip] = ms_due(sc, C) ? ms_flag_one(sc.ms, C.dt, in, ip) : MS_KEEP;
+  partner[ip] = -1;
+}
+
+// One thread per cell: greedy pairing and the min-particles undo; counts
+// per scenario in cnt[4*s + 0..2] (splits, pairs formed, net merges).
+template <typename R>
+__global__ void ens_pair_kernel(Tables T, const int *__restrict__ cell_off,
+                                int nc, MsIn<R> in,
+                                const int *__restrict__ cstart,
+                                const int *__restrict__ cptcls, int *act,
+                                int *partner, int *cnt) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= nc) return;
+  const int b = cstart[cell], e = cstart[cell + 1];
+  if (b == e) return;
+  int lo = 0, hi = T.S - 1;  // last scenario with cell_off <= cell
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cell_off[mid] <= cell) lo = mid;
+    else hi = mid - 1;
+  }
+  const Scen &sc = T.scen[lo];
+  if (!ms_due(sc, T.ctrl[lo])) return;
+  int nsplit = 0;
+  for (int i = b; i < e; ++i) nsplit += act[cptcls[i]] == MS_SPLIT;
+  const int formed =
+      pair_cell(b, e, cptcls, in, sc.ms.max_dv2, act, partner, nullptr, 0);
+  const int undone = undo_cell(b, e, cptcls, sc.ms.min_ppc, act, partner);
+  int *c = cnt + 4 * lo;
+  if (nsplit) atomicAdd(&c[0], nsplit);
+  if (formed) atomicAdd(&c[1], formed);
+  if (formed - undone) atomicAdd(&c[2], formed - undone);
+}
+
+// One workgroup per scenario: where the operation cap could bind (more
+// splits or pairs than ms_max_ops), replay the exact serial loop from fresh
+// flags.
+template <typename R>
+__global__ void ens_serial_kernel(Tables T, MsIn<R> in,
+                                  const int *__restrict__ cstart,
+                                  const int *__restrict__ cptcls, int *act,
+                                  int *partner, int *cnt) {
+  if (threadIdx.x != 0) return;
+  const int sid = blockIdx.x;
+  const Scen &sc = T.scen[sid];
+  const Ctrl &C = T.ctrl[sid];
+  if (!ms_due(sc, C)) return;
+  int *c = cnt + 4 * sid;
+  if (!(c[0] > sc.ms.max_ops || c[1] > sc.ms.max_ops)) return;
+  for (int ip = T.poff[sid]; ip < T.poff[sid + 1]; ++ip) {
+    act[ip] = ms_flag_one(sc.ms, C.dt, in, ip);
+    partner[ip] = -1;
+  }
+  ms_serial_cells(sc.coff, sc.coff + sc.P.nrows * sc.P.ncols, sc.ms, in,
+                  cstart, cptcls, act, partner, c);
+}
+
+__global__ void weights_kernel(const int *__restrict__ poff, int S, int n,
+                               const int *__restrict__ act, int *w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = i < poff[S] ? act[i] : -1;
+  w[i] = a == MS_KEEP ? 1 : a == MS_SPLIT ? 2 : a == MS_PRIMARY ? 1 : 0;
+}
+
+// New particle offsets of every scenario (order is preserved, so scenario
+// s starts at the scan value of its first particle); flags overflow.
+__global__ void ms_offsets_kernel(const int *__restrict__ poff, int S,
+                                  const int *__restrict__ offs, int cap,
+                                  int *poff_next, int *err) {
+  for (int s = threadIdx.x; s <= S; s += blockDim.x)
+    poff_next[s] = offs[poff[s]];
+  if (threadIdx.x == 0 && offs[poff[S]] > cap) *err = 1;
+}
+
+__global__ void ms_commit_kernel(int *poff, const int *__restrict__ poff_next,
+                                 int S, const int *__restrict__ err) {
+  if (*err) return;
+  for (int s = threadIdx.x; s <= S; s += blockDim.x) poff[s] = poff_next[s];
+}
+
+// Scenario of the first particle of every 64-particle wavefront.
+__global__ void wscen_kernel(const int *__restrict__ poff, int S, int nw,
+                             int *wscen) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const int i = w * 64;
+  int lo = 0, hi = S - 1;  // first s with i < poff[s+1]
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (i < poff[mid + 1]) hi = mid;
+    else lo = mid + 1;
+  }
+  wscen[w] = lo;
+}
+
+// Merge/split execution, merge_split_ops_cmes.h:258-427.
+struct ExecArgs {
+  Tables T;
+  PPtr in, out;
+  const int *act, *partner, *offset;
+  const int *err;
+  int ncopy;  // fields copied for kept particles (PF_CORE or PF_COUNT)
+};
+
+template <typename R>
+__device__ __forceinline__ double ldv(const PPtr &p, int k, int i) {
+  return pf_wide(k) ? fld<double>(p, k)[i] : (double)fld<R>(p, k)[i];
+}
+template <typename R>
+__device__ __forceinline__ void stv(const PPtr &p, int k, int i, double v) {
+  if (pf_wide(k)) fld<double>(p, k)[i] = v;
+  else fld<R>(p, k)[i] = (R)v;
+}
+
+template <typename R>
+__global__ void ms_execute_kernel(ExecArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*a.err || i >= a.T.poff[a.T.S]) return;
+  const int act = a.act[i];
+  const int o = a.offset[i];
+  if (act == MS_KEEP) {
+    for (int k = 0; k < a.ncopy; ++k) {
+      if (pf_wide(k)) fld<double>(a.out, k)[o] = fld<double>(a.in, k)[i];
+      else fld<R>(a.out, k)[o] = fld<R>(a.in, k)[i];
+    }
+    a.out.label[o] = a.in.label[i];
+    a.out.level[o] = a.in.level[i];
+    return;
+  }
+  if (act != MS_SPLIT && act != MS_PRIMARY) return;
+  const Scen &sc = a.T.scen[scen_of(a.T, i)];
+  const double Lx = sc.P.ncols * sc.P.hx, Ly = sc.P.nrows * sc.P.hy, eps = 1e-10;
+  const PPtr &in = a.in, &out = a.out;
+  if (act == MS_SPLIT) {
+    const int o1 = o, o2 = o + 1;
+    double ux, uy;
+    const double exx = ldv<R>(in, PF_VXDX, i), eyy = ldv<R>(in, PF_VYDY, i);
+    const double exy = 0.5 * (ldv<R>(in, PF_VXDY, i) + ldv<R>(in, PF_VYDX, i));
+    principal_stretch_dir(exx, eyy, exy, &ux, &uy);
+    const double r_i = sqrt(ldv<R>(in, PF_AP, i));
+    const double off = 0.25 * r_i;
+    const double cx = ldv<R>(in, PF_X, i), cy = ldv<R>(in, PF_Y, i);
+    double x1 = cx + off * ux, y1 = cy + off * uy;
+    double x2 = cx - off * ux, y2 = cy - off * uy;
+    x1 = x1 < eps ? eps : (x1 > Lx - eps ? Lx - eps : x1);
+    y1 = y1 < eps ? eps : (y1 > Ly - eps ? Ly - eps : y1);
+    x2 = x2 < eps ? eps : (x2 > Lx - eps ? Lx - eps : x2);
+    y2 = y2 < eps ? eps : (y2 > Ly - eps ? Ly - eps : y2);
+    for (int k = PF_VPX; k < PF_COUNT; ++k) {
+      const double val = ldv<R>(in, k, i) / (pf_extensive(k) ? 2.0 : 1.0);
+      stv<R>(out, k, o1, val);
+      stv<R>(out, k, o2, val);
+    }
+    const double h = ldv<R>(out, PF_HP, o1);
+    const double Ap1 = ldv<R>(out, PF_VP, o1) / h;
+    stv<R>(out, PF_AP, o1, Ap1);
+    stv<R>(out, PF_AP, o2, ldv<R>(out, PF_VP, o2) / h);
+    stv<R>(out, PF_X, o1, x1);
+    stv<R>(out, PF_Y, o1, y1);
+    stv<R>(out, PF_X, o2, x2);
+    stv<R>(out, PF_Y, o2, y2);
+    const double Mc = ldv<R>(out, PF_MP, o1);
+    const double mx = Mc * ldv<R>(out, PF_VPX, o1);
+    const double my = Mc * ldv<R>(out, PF_VPY, o1);
+    stv<R>(out, PF_MOMX, o1, mx);
+    stv<R>(out, PF_MOMX, o2, mx);
+    stv<R>(out, PF_MOMY, o1, my);
+    stv<R>(out, PF_MOMY, o2, my);
+    const double Fbx = ldv<R>(out, PF_FBX, o1), Fby = ldv<R>(out, PF_FBY, o1);
+    stv<R>(out, PF_FRX, o1, Ap1 * Fbx);
+    stv<R>(out, PF_FRX, o2, Ap1 * Fbx);
+    stv<R>(out, PF_FRY, o1, Ap1 * Fby);
+    stv<R>(out, PF_FRY, o2, Ap1 * Fby);
+    out.label[o1] = out.label[o2] = -1;
+    out.level[o1] = out.level[o2] = in.level[i] - 1;
+  } else {
+    const int j = a.partner[i];
+    const double M1 = ldv<R>(in, PF_MP, i), M2 = ldv<R>(in, PF_MP, j);
+    const double Mt = M1 + M2;
+    const double nx = (M1 * ldv<R>(in, PF_X, i) + M2 * ldv<R>(in, PF_X, j)) / Mt;
+    const double ny = (M1 * ldv<R>(in, PF_Y, i) + M2 * ldv<R>(in, PF_Y, j)) / Mt;
+    for (int k = PF_VPX; k < PF_COUNT; ++k)
+      stv<R>(out, k, o,
+             pf_extensive(k)
+                 ? (ldv<R>(in, k, i) + ldv<R>(in, k, j))
+                 : (M1 * ldv<R>(in, k, i) + M2 * ldv<R>(in, k, j)) / Mt);
+    stv<R>(out, PF_HP, o, ldv<R>(out, PF_VP, o) / ldv<R>(out, PF_AP, o));
+    stv<R>(out, PF_X, o, nx);
+    stv<R>(out, PF_Y, o, ny);
+    const double Mp = ldv<R>(out, PF_MP, o);
+    stv<R>(out, PF_MOMX, o, Mp * ldv<R>(out, PF_VPX, o));
+    stv<R>(out, PF_MOMY, o, Mp * ldv<R>(out, PF_VPY, o));
+    const double Ap = ldv<R>(out, PF_AP, o);
+    stv<R>(out, PF_FRX, o, Ap * ldv<R>(out, PF_FBX, o));
+    stv<R>(out, PF_FRY, o, Ap * ldv<R>(out, PF_FBY, o));
+    out.label[o] = -1;
+    out.level[o] = in.level[i] + 1;
+  }
+}
+
+// ------------------------------------------------------------- fills
+__global__ void fill_f64_kernel(double *p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_f32_kernel(float *p, long long n, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_int_kernel(int *p, long long n, int v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+// Narrow a host-uploaded FP64 particle field to FP32.
+__global__ void narrow_kernel(const double *src, float *dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+__global__ void widen_kernel(const float *src, double *dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (double)src[i];
+}
+
+}  // namespace amsc

@@ -128,6 +128,23 @@ def lib():
     L.amsc_sim_conservation.argtypes = [_vp, _dp]
     L.amsc_sim_ms_counts.argtypes = [_vp, _ip]
     L.amsc_sim_particle_steps.argtypes = [_vp, C.POINTER(C.c_longlong)]
+    L.amsc_sim_set_graph.argtypes = [_vp, C.c_int]
+    L.amsc_sim_set_tiling.argtypes = [_vp, C.c_int, C.c_int]
+    L.amsc_sim_autotune.argtypes = [_vp, C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.amsc_ens_create.argtypes = [C.c_int, C.POINTER(AmscParams), _ip] + \
+        [_dp] * 11 + [C.c_int, C.c_longlong, C.POINTER(_vp)]
+    L.amsc_ens_status.argtypes = [_vp, _ip, _dp, _dp, _ip, _ip]
+    L.amsc_ens_get_field.argtypes = [_vp, C.c_int, C.c_char_p, _dp, C.c_int]
+    L.amsc_ens_get_field_int.argtypes = [_vp, C.c_int, C.c_char_p, _ip, C.c_int]
+    L.amsc_ens_set_field.argtypes = [_vp, C.c_int, C.c_char_p, _dp, C.c_int]
+    L.amsc_ens_conservation.argtypes = [_vp, C.c_int, _dp]
+    L.amsc_ens_ms_counts.argtypes = [_vp, C.c_int, _ip]
+    L.amsc_ens_info.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+    L.amsc_ens_bytes_estimate.argtypes = [C.c_int, C.POINTER(AmscParams), _ip,
+                                          C.c_int, C.c_longlong]
+    L.amsc_ens_bytes_estimate.restype = C.c_longlong
     _lib = L
     return L
 

@@ -1,6 +1,7 @@
 """Host-side driver over the C ABI: the role of main/gpu_offload.cpp (load
 DATA, run the time loop, read fields back), with all state resident on the
-GPU between calls.
+GPU between calls; `Ensemble` advances a batch of scenarios together, in
+FP64 or mixed FP32/FP64.
 """
 from __future__ import annotations
 
@@ -10,6 +11,8 @@ import numpy as np
 
 from . import native
 from .data import load_data_json, params_from_data, particles_from_data
+
+PRECISION = {"fp64": 0, "mixed": 1}
 
 NODAL_FIELDS = ("vvxL", "vvyL", "avx", "avy", "Mv", "mom_vx", "mom_vy",
                 "F_ext_vx", "F_ext_vy", "F_int_vx", "F_int_vy", "Fric_x",
@@ -144,3 +147,117 @@ class Simulation:
         out = np.zeros(2, np.int32)
         native.check(native.lib().amsc_sim_ms_counts(self._h, out))
         return out
+
+    def set_graph(self, on: bool = True):
+        """Replay each step variant from a HIP graph recorded once."""
+        native.check(native.lib().amsc_sim_set_graph(self._h, int(bool(on))))
+
+    def set_tiling(self, p2g_shape: int, particle_block: int):
+        native.check(native.lib().amsc_sim_set_tiling(self._h, int(p2g_shape),
+                                                      int(particle_block)))
+
+    def autotune(self, steps_per_variant: int = 2):
+        """Pick the fastest P2G tile shape and particle workgroup size on live
+        steps; returns (shape, block, steps used)."""
+        a, b, u = C.c_int(), C.c_int(), C.c_int()
+        native.check(native.lib().amsc_sim_autotune(self._h, int(steps_per_variant),
+                                                    C.byref(a), C.byref(b),
+                                                    C.byref(u)))
+        return a.value, b.value, u.value
+
+
+class Ensemble(Simulation):
+    """A batch of independent scenarios advanced together (amsc_ens_*).
+
+    scenarios: list of (prm, particles, Z) as for Simulation.  precision
+    "fp64" (bit-identical to running each scenario alone) or "mixed".
+    """
+
+    def __init__(self, scenarios, precision: str = "fp64", capacity: int = 0):
+        L = native.lib()
+        self.nscen = len(scenarios)
+        self.prms = [dict(p) for p, _, _ in scenarios]
+        self.prm = self.prms[0]
+        self.precision = precision
+        P = (native.AmscParams * self.nscen)(
+            *[native.AmscParams.from_dict(p) for p in self.prms])
+        nps = np.array([len(part["x"]) for _, part, _ in scenarios], np.int32)
+        cat = [np.ascontiguousarray(np.concatenate(
+            [np.asarray(part[k], np.float64) for _, part, _ in scenarios]))
+            for k in ("x", "y", "Mp", "Ap", "vpx", "vpy", "mom_px", "mom_py",
+                      "hp", "Vp")]
+        Z = np.ascontiguousarray(np.concatenate(
+            [np.asarray(z, np.float64).ravel() for _, _, z in scenarios]))
+        h = C.c_void_p()
+        native.check(L.amsc_ens_create(self.nscen, P, nps, *cat, Z,
+                                       PRECISION[precision], int(capacity),
+                                       C.byref(h)))
+        self._h = h
+        self.nns = [(p["nrows"] + 1) * (p["ncols"] + 1) for p in self.prms]
+        self.ncs = [p["nrows"] * p["ncols"] for p in self.prms]
+        self.nn, self.nc = self.nns[0], self.ncs[0]
+
+    @classmethod
+    def from_data(cls, ds, precision: str = "fp64", capacity: int = 0,
+                  max_steps: int = 0):
+        sc = []
+        for d in ds:
+            prm = params_from_data(d, max_steps=max_steps or int(d.get("MAX_STEPS", 0)))
+            sc.append((prm, particles_from_data(d), d["Z"]))
+        return cls(sc, precision, capacity)
+
+    def status(self) -> dict:
+        n = self.nscen
+        it, act, npart = (np.zeros(n, np.int32) for _ in range(3))
+        t, dt = np.zeros(n), np.zeros(n)
+        native.check(native.lib().amsc_ens_status(self._h, it, t, dt, act, npart))
+        return {"it": it, "t": t, "dt": dt, "active": act, "np": npart}
+
+    def num(self, scen: int) -> int:
+        return int(self.status()["np"][scen])
+
+    def get(self, scen: int, name: str) -> np.ndarray:
+        n = self.nns[scen] if name in NODAL_FIELDS else self.num(scen)
+        out = np.empty(max(n, 1), np.float64)
+        native.check(native.lib().amsc_ens_get_field(self._h, int(scen),
+                                                     name.encode(), out, len(out)))
+        return out[:n]
+
+    def get_int(self, scen: int, name: str) -> np.ndarray:
+        n = self.ncs[scen] + 1 if name == "cstart" else self.num(scen)
+        out = np.empty(max(n, 1), np.int32)
+        native.check(native.lib().amsc_ens_get_field_int(self._h, int(scen),
+                                                         name.encode(), out, len(out)))
+        return out[:n]
+
+    def set(self, scen: int, name: str, arr):
+        a = np.ascontiguousarray(arr, np.float64)
+        native.check(native.lib().amsc_ens_set_field(self._h, int(scen),
+                                                     name.encode(), a, len(a)))
+
+    def conservation(self, scen: int = 0) -> np.ndarray:
+        out = np.zeros(3, np.float64)
+        native.check(native.lib().amsc_ens_conservation(self._h, int(scen), out))
+        return out
+
+    def ms_counts(self, scen: int = 0) -> np.ndarray:
+        out = np.zeros(2, np.int32)
+        native.check(native.lib().amsc_ens_ms_counts(self._h, int(scen), out))
+        return out
+
+    def info(self) -> dict:
+        n, p, b, c = C.c_int(), C.c_int(), C.c_longlong(), C.c_int()
+        native.check(native.lib().amsc_ens_info(self._h, C.byref(n), C.byref(p),
+                                                C.byref(b), C.byref(c)))
+        return {"nscen": n.value, "precision": p.value, "bytes": b.value,
+                "capacity": c.value}
+
+    @staticmethod
+    def bytes_estimate(prms, nps, precision: str = "fp64", capacity: int = 0) -> int:
+        P = (native.AmscParams * len(prms))(*[native.AmscParams.from_dict(p) for p in prms])
+        v = native.lib().amsc_ens_bytes_estimate(len(prms), P,
+                                                 np.asarray(nps, np.int32),
+                                                 PRECISION[precision], int(capacity))
+        if v < 0:
+            raise ValueError("bad arguments")
+        return int(v)

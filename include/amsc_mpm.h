@@ -6,13 +6,16 @@
  * Launches are stream-ordered on the stream set with amsc_set_stream
  * (default: the null stream) and do not synchronise unless stated.
  *
- * Three groups of entry points:
+ * Four groups of entry points:
  *   1. the 18 step kernels of the reference's gpu_kernels.h, same parameter
  *      lists (device pointers), same results;
  *   2. device binning (replaces quadgrid's host init_particle_mesh + the
  *      per-step host CSR rebuild) and merge/split decisions on the device;
  *   3. amsc_sim_*: a device-resident time loop (the gpu_offload.cpp driver's
- *      job) built on fused kernels, FP64 with optim_par semantics.
+ *      job) built on fused kernels, FP64 with optim_par semantics;
+ *   4. amsc_ens_*: the same loop over a batch of independent scenarios
+ *      advanced together, in FP64 or mixed FP32/FP64 (no reference
+ *      counterpart: the hazard-ensemble form of the time loop).
  */
 #ifndef AMSC_MPM_H
 #define AMSC_MPM_H
@@ -277,6 +280,61 @@ int amsc_sim_conservation(amsc_sim *s, double out[3]);
 int amsc_sim_particle_steps(amsc_sim *s, long long *out);
 /* splits, merges of the last merge/split call. */
 int amsc_sim_ms_counts(amsc_sim *s, int out[2]);
+/* 1: record every step variant once into a HIP graph and replay it (launch
+ * extents then follow the particle capacity; results are unchanged).
+ * Ignored while per-kernel timing is on. */
+int amsc_sim_set_graph(amsc_sim *s, int on);
+/* P2G tile shape (0: 16x32 nodes, 1: 16x16, 2: 8x32) and particle-kernel
+ * workgroup size (256 or 512); results do not depend on either. */
+int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block);
+/* Autotuner: time every tile shape, then every workgroup size, over
+ * steps_per_variant live steps each and keep the fastest.  The steps are
+ * real steps of the run; *steps_used (may be NULL) receives their number. */
+int amsc_sim_autotune(amsc_sim *s, int steps_per_variant, int *p2g_shape,
+                      int *particle_block, int *steps_used);
+
+/* ------------------------------------------------------------------ 4. */
+#define AMSC_PREC_FP64 0  /* every particle/nodal value and operation FP64 */
+#define AMSC_PREC_MIXED 1 /* FP32 arithmetic; FP64 positions, depth, H, the
+                             free-surface difference, P2G mass/momentum sums
+                             and conservation reductions */
+
+/* An ensemble is a simulation object holding nscen scenarios; every
+ * amsc_sim_* function accepts it and then refers to scenario 0. */
+typedef struct amsc_sim amsc_ens;
+
+/* Create nscen scenarios: params[i], np[i] particles; the particle arrays
+ * are the scenarios' DATA arrays concatenated in scenario order (sum of np
+ * entries each), Z the concatenated nodal Z arrays.  capacity bounds the
+ * total particle count merge/split may reach (0 = twice the initial). */
+int amsc_ens_create(int nscen, const amsc_params *params, const int *np,
+                    const double *x, const double *y, const double *Mp,
+                    const double *Ap, const double *vpx, const double *vpy,
+                    const double *mom_px, const double *mom_py,
+                    const double *hp, const double *Vp, const double *Z,
+                    int precision, long long capacity, amsc_ens **out);
+/* Per-scenario steps taken, t, dt of the last step, 1 while the run goes
+ * on, particle count; arrays of nscen entries, any may be NULL.
+ * Synchronises.  amsc_sim_step advances every scenario that is still
+ * running; *done is the largest number of steps any scenario took. */
+int amsc_ens_status(amsc_ens *e, int *it, double *t, double *dt, int *active,
+                    int *np);
+int amsc_ens_get_field(amsc_ens *e, int scen, const char *name, double *out,
+                       int n);
+int amsc_ens_get_field_int(amsc_ens *e, int scen, const char *name, int *out,
+                           int n);
+int amsc_ens_set_field(amsc_ens *e, int scen, const char *name,
+                       const double *in, int n);
+int amsc_ens_conservation(amsc_ens *e, int scen, double out[3]);
+int amsc_ens_ms_counts(amsc_ens *e, int scen, int out[2]);
+/* Scenarios, precision, device bytes held, particle capacity. */
+int amsc_ens_info(amsc_ens *e, int *nscen, int *precision, long long *bytes,
+                  int *capacity);
+/* Device bytes amsc_ens_create would allocate (admission control); -1 on
+ * bad arguments. */
+long long amsc_ens_bytes_estimate(int nscen, const amsc_params *params,
+                                  const int *np, int precision,
+                                  long long capacity);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,295 @@
+"""Batches of scenarios advanced together (amsc_ens_*), graph replay, tile
+shapes, and the mixed-precision form.
+
+* FP64 batches: every scenario bit-for-bit equal to its own oracle run
+  (positions, velocities, depths, stresses, friction, dt, t, particle counts
+  after merge/split), whatever the other scenarios of the batch do.
+* Graph replay and every P2G tile shape / particle workgroup size give the
+  same bits as eager launches.
+* Mixed FP32/FP64: each scenario of a mixed batch is bit-identical to the
+  same scenario run alone in mixed precision; against FP64 the depth stays
+  within 1e-4 relative (the target of the mixed form) and the velocity
+  within 1e-3 of the flow speed after 40 steps; mass is conserved to the
+  1e-10 threshold of data/conservation_test/scripts/conservation_check.py:36
+  (FP64 mass sums); a lake at rest stays at rest (L-inf drift of the free
+  surface below the 1e-10 of data/well_balance/wb_error.py:72).
+"""
+import numpy as np
+import pytest
+
+from amsc_amd import native, scenarios
+from amsc_amd.data import params_from_data, particles_from_data
+from oracle import oracle
+from test_oracle_step import small_slope_case
+
+pytestmark = pytest.mark.gpu
+
+# (Fric_p is stored one step ahead by the fused loop: it is the next P2G's
+# friction force, Ap*Fb of this step, while optim_par forms it at the start
+# of the next step; it is compared through Ap and Fb.)
+FIELDS = ("x", "y", "vpx", "vpy", "hp", "Vp", "Mp", "H", "F_11", "F_12",
+          "F_22", "Ap", "Fb_x", "Fb_y", "vpx_dx", "vpy_dy", "Zp", "mom_px")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.int64)
+
+
+def scen(d, max_steps=0, **over):
+    prm = params_from_data(d, max_steps=max_steps)
+    prm.update(over)
+    return prm, particles_from_data(d), d["Z"]
+
+
+def mixed_cases():
+    """Four small scenarios of different grids, sizes, time stepping and
+    merge/split settings."""
+    a = small_slope_case()
+    b = small_slope_case(seed=9, nrows=14, ncols=11, hx=0.4, hy=0.5)
+    b.update(MERGE_SPLIT_ON=1, ms_call_interval=4, ms_alpha=0.3, ms_beta=0.52,
+             ms_max_dv=2.0, ms_hp_min=0.0)
+    c = scenarios.ensemble_scenario(11, side_range=(24, 40), np_range=(400, 1500))
+    d = scenarios.dambreak_flat(30)
+    return [scen(a), scen(b), scen(c, max_steps=30), scen(d)]
+
+
+def assert_scen_equal(e, k, o, where):
+    assert e.num(k) == o.np, where
+    for f in FIELDS:
+        a, b = e.get(k, f), o.get(f)
+        assert np.array_equal(bits(a), bits(b)), f"{where}: scenario {k} {f}"
+    assert np.array_equal(e.get_int(k, "level"), o.get_int("level")), where
+    st = e.status()
+    assert (int(st["it"][k]), st["t"][k], st["dt"][k]) == (o.it, o.t, o.dt), where
+
+
+def test_fp64_batch_matches_each_oracle():
+    from amsc_amd.simulation import Ensemble
+    print("[ens] fp64 batch vs oracle", flush=True)
+    cases = mixed_cases()
+    e = Ensemble(cases, "fp64", capacity=8 * sum(len(p["x"]) for _, p, _ in cases))
+    e.set_aux(True)
+    orc = [oracle.OracleSim(prm, part, Z) for prm, part, Z in cases]
+    for step in range(1, 31):
+        e.step(1)
+        for o in orc:
+            o.step()
+        if step in (1, 4, 12, 20, 30):
+            for k, o in enumerate(orc):
+                assert_scen_equal(e, k, o, f"step {step}")
+                em, eo, _ = e.conservation(k)
+                assert em < 1e-10 and eo < 1e-10
+    # scenario 1 did split and merge, scenario 3 ended at NSTEPS = 30
+    assert e.status()["np"][1] != len(cases[1][1]["x"])
+    assert e.step(3) == 3 and e.status()["it"][3] == 30
+    e.close()
+
+
+def test_fp64_batch_without_outputs_matches_oracle():
+    """Output fields off (only the step before a merge/split writes what the
+    decision reads): still the oracle's state after 29 steps."""
+    from amsc_amd.simulation import Ensemble
+    print("[ens] fp64 batch, outputs off", flush=True)
+    cases = mixed_cases()
+    e = Ensemble(cases, "fp64", capacity=8 * sum(len(p["x"]) for _, p, _ in cases))
+    orc = [oracle.OracleSim(prm, part, Z) for prm, part, Z in cases]
+    e.step(28)
+    e.set_aux(True)
+    e.step(1)
+    for o in orc:
+        o.run(29)
+    for k, o in enumerate(orc):
+        assert_scen_equal(e, k, o, "step 29")
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_matches_eager(precision):
+    from amsc_amd.simulation import Ensemble
+    print(f"[ens] graph vs eager {precision}", flush=True)
+    cases = mixed_cases()
+    cap = 8 * sum(len(p["x"]) for _, p, _ in cases)
+    a = Ensemble(cases, precision, capacity=cap)
+    b = Ensemble(cases, precision, capacity=cap)
+    b.set_graph(True)
+    for s in (a, b):
+        s.step(13)
+        s.set_aux(True)
+        s.step(12)
+    for k in range(len(cases)):
+        assert a.num(k) == b.num(k)
+        for f in FIELDS:
+            assert np.array_equal(bits(a.get(k, f)), bits(b.get(k, f))), (k, f)
+        assert np.array_equal(a.ms_counts(k), b.ms_counts(k))
+    for f in ("it", "t", "dt", "np"):
+        assert np.array_equal(a.status()[f], b.status()[f]), f
+    a.close()
+    b.close()
+
+
+def test_tile_shapes_and_blocks_give_identical_results():
+    from amsc_amd.simulation import Simulation
+    print("[ens] tilings", flush=True)
+    d = small_slope_case(nrows=40, ncols=37, hx=0.25, hy=0.3)
+    ref = None
+    for shape in (0, 1, 2):
+        for blk in (256, 512):
+            s = Simulation.from_data(d)
+            s.set_tiling(shape, blk)
+            s.step(8)
+            s.set_aux(True)
+            s.step(1)
+            got = {f: s.get(f) for f in FIELDS}
+            got["vvxL"] = s.get("vvxL")
+            if ref is None:
+                ref = got
+            for f in got:
+                assert np.array_equal(bits(got[f]), bits(ref[f])), (shape, blk, f)
+            s.close()
+
+
+def test_autotune_keeps_oracle_parity():
+    from amsc_amd.simulation import Simulation
+    print("[ens] autotune", flush=True)
+    d = small_slope_case()
+    s = Simulation.from_data(d)
+    shape, blk, used = s.autotune(2)
+    assert shape in (0, 1, 2) and blk in (256, 512) and used == 10
+    o = oracle.OracleSim(params_from_data(d), particles_from_data(d), d["Z"])
+    for _ in range(used + 3):
+        o.step()
+    s.set_aux(True)
+    s.step(3)
+    for f in FIELDS:
+        assert np.array_equal(bits(s.get(f)), bits(o.get(f))), f
+    s.close()
+
+
+def test_mixed_batch_equals_mixed_single_runs():
+    from amsc_amd.simulation import Ensemble
+    print("[ens] mixed batch vs single", flush=True)
+    cases = mixed_cases()
+    cap = 8 * sum(len(p["x"]) for _, p, _ in cases)
+    e = Ensemble(cases, "mixed", capacity=cap)
+    e.set_aux(True)
+    e.step(20)
+    for k, c in enumerate(cases):
+        s = Ensemble([c], "mixed", capacity=8 * len(c[1]["x"]))
+        s.set_aux(True)
+        s.step(20)
+        assert s.num(0) == e.num(k)
+        for f in FIELDS:
+            assert np.array_equal(bits(s.get(0, f)), bits(e.get(k, f))), (k, f)
+        s.close()
+    e.close()
+
+
+def rel_depth(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("seed,adaptive,steps", [(5, False, 80), (7, False, 80),
+                                                 (7, True, 40)])
+def test_mixed_precision_error_against_fp64(seed, adaptive, steps):
+    """Mixed vs FP64 from the same state: relative depth error <= 1e-4 (max
+    over particles / max depth), velocity error <= 1e-2 of the gravity-wave
+    speed sqrt(g hmax) at the worst particle and <= 1e-5 at the median (the
+    worst particles sit in the near-rigid Bingham plug, where the stress
+    tau_Y/sqrt(II_D) amplifies rounding), mass conserved to 1e-10 every
+    step."""
+    from amsc_amd.simulation import Ensemble
+    print(f"[ens] mixed vs fp64 seed {seed} adaptive {adaptive}", flush=True)
+    d = scenarios.ensemble_scenario(seed, side_range=(48, 64), np_range=(3000, 6000))
+    over = {"ms_on": 0} if adaptive else {"ms_on": 0, "dt_fixed": 1e-3,
+                                          "nsteps": 1000}
+    c = scen(d, max_steps=1000, **over)
+    m = Ensemble([c], "mixed")
+    f = Ensemble([c], "fp64")
+    worst = 0.0
+    for _ in range(steps):
+        m.step(1)
+        f.step(1)
+        worst = max(worst, m.conservation(0)[0])
+    assert worst < 1e-10
+    hm, hd = m.get(0, "hp"), f.get(0, "hp")
+    dv = np.hypot(m.get(0, "vpx") - f.get(0, "vpx"), m.get(0, "vpy") - f.get(0, "vpy"))
+    cw = np.sqrt(9.81 * hd.max())
+    print(f"[ens] seed {seed} adaptive {adaptive} after {steps} steps: depth "
+          f"{rel_depth(hm, hd):.3e} (median {np.median(np.abs(hm - hd)) / hd.max():.2e}),"
+          f" velocity {dv.max() / cw:.3e} (median {np.median(dv) / cw:.2e}) of "
+          f"sqrt(g h)", flush=True)
+    assert rel_depth(hm, hd) <= 1e-4
+    assert dv.max() / cw <= 1e-2 and np.median(dv) / cw <= 1e-5
+    m.close()
+    f.close()
+
+
+def test_mixed_error_distribution_on_a_sensitive_flow():
+    """Adaptive CFL 0.4 on a flow with near-rigid zones: there the
+    unregularised Bingham stress (tau_Y/sqrt(II_D) + 2 mu) D of
+    optim_par.cpp:68-120 has the full yield magnitude in a direction set by
+    rounding, so FP32 and FP64 differ by O(tau_Y) in those particles' stress
+    and the few affected particles depart quickly.  The bulk of the flow
+    stays with FP64: median relative depth error <= 1e-5 after 41 steps."""
+    from amsc_amd.simulation import Ensemble
+    print("[ens] sensitive flow", flush=True)
+    d = scenarios.ensemble_scenario(5, side_range=(48, 64), np_range=(3000, 6000))
+    c = scen(d, max_steps=1000, ms_on=0)
+    m, f = Ensemble([c], "mixed"), Ensemble([c], "fp64")
+    m.step(41)
+    f.step(41)
+    hm, hd = m.get(0, "hp"), f.get(0, "hp")
+    e = np.abs(hm - hd) / hd.max()
+    print(f"[ens] sensitive flow after 41 steps: depth error max {e.max():.2e}, "
+          f"p99 {np.percentile(e, 99):.2e}, p90 {np.percentile(e, 90):.2e}, "
+          f"median {np.median(e):.2e}", flush=True)
+    assert np.median(e) <= 1e-5
+    m.close()
+    f.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lake_at_rest(precision):
+    from amsc_amd.simulation import Ensemble
+    print(f"[ens] lake at rest {precision}", flush=True)
+    d = scenarios.large_fractal(nex=64, nsteps=200, still_water=True)
+    e = Ensemble([scen(d)], precision)
+    e.step(199)
+    e.set_aux(True)
+    e.step(1)
+    dh = np.abs(e.get(0, "hp") - d["hp"]).max()
+    dx = max(np.abs(e.get(0, "x") - d["x"]).max(), np.abs(e.get(0, "y") - d["y"]).max())
+    vmax = max(np.abs(e.get(0, "vpx")).max(), np.abs(e.get(0, "vpy")).max())
+    print(f"[ens] lake at rest {precision}: |dh| {dh:.3e} m, |dx| {dx:.3e} m, "
+          f"|v| {vmax:.3e} m/s", flush=True)
+    assert dh < 1e-10 and dx < 1e-10 and vmax < 1e-10
+    if precision == "fp64":
+        surf = e.get(0, "hp") + e.get(0, "Zp")
+        assert np.abs(surf - 10.0).max() < 1e-10
+    e.close()
+
+
+def test_capacity_overflow_is_reported():
+    from amsc_amd.simulation import Ensemble
+    print("[ens] capacity", flush=True)
+    d = small_slope_case()
+    d.update(MERGE_SPLIT_ON=1, ms_call_interval=2, ms_alpha=0.1, ms_beta=0.0,
+             ms_hp_min=0.0)
+    c = scen(d)
+    e = Ensemble([c], "fp64", capacity=len(c[1]["x"]) + 3)
+    with pytest.raises(native.AmscError) as ei:
+        e.step(4)
+    assert ei.value.code == 4
+    e.close()
+
+
+def test_bytes_estimate_covers_allocation():
+    from amsc_amd.simulation import Ensemble
+    cases = mixed_cases()
+    for prec in ("fp64", "mixed"):
+        est = Ensemble.bytes_estimate([p for p, _, _ in cases],
+                                      [len(q["x"]) for _, q, _ in cases], prec)
+        e = Ensemble(cases, prec)
+        held = e.info()["bytes"]
+        assert held <= est <= 1.2 * held + (1 << 20), (prec, held, est)
+        e.close()
