@@ -51,9 +51,11 @@ static const char *kNFName[NF_COUNT] = {
     "Fric_y", "FPxv",     "FPyv",     "Ftot_vx",  "Ftot_vy",  "vvx",
     "vvy"};
 
-// P2G tile shapes (node columns x node rows, one thread per node) and
-// particle-kernel workgroup sizes the autotuner chooses from.
+// P2G tile shapes (node columns x node rows, one thread per node), tile
+// orders (XCD-grouped or linear) and particle-kernel workgroup sizes the
+// autotuner chooses from.  A P2G variant v is shape v % 3, order v / 3.
 constexpr int kTileCfgs = 3;
+constexpr int kP2GVariants = 2 * kTileCfgs;
 constexpr int kTileTC[kTileCfgs] = {16, 16, 8};
 constexpr int kTileTR[kTileCfgs] = {32, 16, 32};
 constexpr int kPartBS[2] = {256, 512};
@@ -102,7 +104,7 @@ struct Sim {
   std::vector<int> ms_counts;  // splits, merges of the last call, per scenario
   std::vector<int> it_host;    // steps launched per scenario
   std::vector<char> ended;     // run known to have ended, per scenario
-  int tile_cfg = 0, part_bs = 256;
+  int tile_cfg = 0, tile_order = 0, part_bs = 256;
   long long particle_steps = 0;
   size_t bytes = 0;  // device memory held
   bool finished = false, aux_on = false, diag = true, mom_stored = false;
@@ -417,17 +419,53 @@ int merge_split(Sim &s, bool capture) {
   return rebin(s, capture);
 }
 
-template <typename R>
-int launch_p2g(Sim &s, const P2GArgs &a) {
+template <typename R, bool XCD>
+int launch_p2g_order(Sim &s, const P2GArgs &a) {
   const dim3 g(s.ntiles[s.tile_cfg]);
   switch (s.tile_cfg) {
   case 0:
-    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 32>, g, dim3(512), a);
+    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 32, XCD>, g, dim3(512), a);
   case 1:
-    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 16>, g, dim3(256), a);
+    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 16, XCD>, g, dim3(256), a);
   default:
-    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 8, 32>, g, dim3(256), a);
+    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 8, 32, XCD>, g, dim3(256), a);
   }
+}
+
+template <typename R>
+int launch_p2g(Sim &s, const P2GArgs &a) {
+  return s.tile_order == 0 ? launch_p2g_order<R, true>(s, a)
+                           : launch_p2g_order<R, false>(s, a);
+}
+
+P2GArgs p2g_args(Sim &s, int aux, bool partials) {
+  PPtr &p = s.buf[s.cur];
+  P2GArgs a;
+  a.T = tables(s);
+  a.tscen = s.tscen[s.tile_cfg];
+  a.aux = aux;
+  a.mom_stored = s.mom_stored ? 1 : 0;
+  a.cstart = s.bin.cstart;
+  a.cptcls = s.cptcls;
+  a.x = fld<double>(p, PF_X);
+  a.y = fld<double>(p, PF_Y);
+  a.hp = fld<double>(p, PF_HP);
+  a.H = fld<double>(p, PF_H);
+  a.Mp = p.f[PF_MP];
+  a.m1 = s.mom_stored ? p.f[PF_MOMX] : p.f[PF_VPX];
+  a.m2 = s.mom_stored ? p.f[PF_MOMY] : p.f[PF_VPY];
+  a.Vp = p.f[PF_VP];
+  a.frx = p.f[PF_FRX];
+  a.fry = p.f[PF_FRY];
+  a.f11 = p.f[PF_F11];
+  a.f12 = p.f[PF_F12];
+  a.f22 = p.f[PF_F22];
+  a.Z = s.Z;
+  a.vvxL = s.nod.f[NF_VVXL];
+  a.vvyL = s.nod.f[NF_VVYL];
+  a.partials = partials ? s.partials : nullptr;
+  a.out = s.p2g_out[s.cur];
+  return a;
 }
 
 template <typename R, int AUX, int BS>
@@ -471,34 +509,7 @@ int step_body(Sim &s, bool ms_step, int aux, bool capture, bool *skipped) {
   }
   const Tables T = tables(s);
   PPtr &p = s.buf[s.cur];
-  {
-    P2GArgs a;
-    a.T = T;
-    a.tscen = s.tscen[s.tile_cfg];
-    a.aux = aux;
-    a.mom_stored = s.mom_stored ? 1 : 0;
-    a.cstart = s.bin.cstart;
-    a.cptcls = s.cptcls;
-    a.x = fld<double>(p, PF_X);
-    a.y = fld<double>(p, PF_Y);
-    a.hp = fld<double>(p, PF_HP);
-    a.H = fld<double>(p, PF_H);
-    a.Mp = p.f[PF_MP];
-    a.m1 = s.mom_stored ? p.f[PF_MOMX] : p.f[PF_VPX];
-    a.m2 = s.mom_stored ? p.f[PF_MOMY] : p.f[PF_VPY];
-    a.Vp = p.f[PF_VP];
-    a.frx = p.f[PF_FRX];
-    a.fry = p.f[PF_FRY];
-    a.f11 = p.f[PF_F11];
-    a.f12 = p.f[PF_F12];
-    a.f22 = p.f[PF_F22];
-    a.Z = s.Z;
-    a.vvxL = s.nod.f[NF_VVXL];
-    a.vvyL = s.nod.f[NF_VVYL];
-    a.partials = s.diag ? s.partials : nullptr;
-    a.out = s.p2g_out[s.cur];
-    if (int rc = launch_p2g<R>(s, a)) return rc;
-  }
+  if (int rc = launch_p2g<R>(s, p2g_args(s, aux, s.diag))) return rc;
   if (s.diag)
     if (int rc = launch(s, "conservation", conservation_kernel, dim3(s.S),
                         dim3(256), (const Scen *)s.scen[s.tile_cfg],
@@ -537,7 +548,8 @@ int launch_graph(Sim &s, bool ms_step, int aux) {
   const long long gk = (long long)ms_step | (long long)aux << 1 |
                        (long long)s.cur << 3 | (long long)s.kcur << 4 |
                        (long long)s.tile_cfg << 5 |
-                       (long long)(s.part_bs == 512) << 8 |
+                       (long long)s.tile_order << 7 |
+                       (long long)(s.part_bs == 512) << 12 |
                        (long long)s.diag << 9 | (long long)s.mom_stored << 10 |
                        (long long)s.aux_on << 11;
   auto it = s.graphs.find(gk);
@@ -674,13 +686,11 @@ int upload_narrow(Sim &s, void *dst, const double *src, size_t n,
                 (const double *)stage, static_cast<float *>(dst), (long long)n);
 }
 
-int create(Sim *s, int S, const amsc_params *prm, const int *np_arr,
-           const double *x, const double *y, const double *Mp,
-           const double *Ap, const double *vpx, const double *vpy,
-           const double *mom_px, const double *mom_py, const double *hp,
-           const double *Vp, const double *Z, int precision,
-           long long capacity) {
-  if (S <= 0 || S > kMaxScen || !prm || !np_arr || !Z)
+// Layout, scenario tables and device allocations of a batch; zero nodal
+// arrays.  Particle data, Z and the control blocks are filled by the caller.
+int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
+          int precision, long long capacity) {
+  if (S <= 0 || S > kMaxScen || !prm || !np_arr)
     return fail(AMSC_ERR_INVALID, "bad ensemble arguments");
   if (precision != AMSC_PREC_FP64 && precision != AMSC_PREC_MIXED)
     return fail(AMSC_ERR_INVALID, "unknown precision");
@@ -707,9 +717,6 @@ int create(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   s->poff_h[S] = (int)npt;
   s->coff_h[S] = (int)nct;
   s->noff_h[S] = (int)nnt;
-  if (npt > 0 && (!x || !y || !Mp || !Ap || !vpx || !vpy || !mom_px ||
-                  !mom_py || !hp || !Vp))
-    return fail(AMSC_ERR_INVALID, "NULL particle array");
   s->np = (int)npt;
   long long cap = capacity > 0 ? capacity : 2 * (npt > 0 ? npt : 1);
   if (cap < npt) cap = npt;
@@ -795,9 +802,42 @@ int create(Sim *s, int S, const amsc_params *prm, const int *np_arr,
       o.nod[k] = nf_narrow(k) ? nullptr : static_cast<double *>(s->nod.f[k]);
     AMSC_HIP(hipMemcpy(s->p2g_out[b], &o, sizeof(o), hipMemcpyHostToDevice));
   }
+  for (int k = 0; k < NF_COUNT; ++k) {
+    int rc;
+    if (nf_bytes(*s, k) == 8)
+      rc = launch(*s, "fill", fill_f64_kernel, dim3(grid_for(s->nn, 256)),
+                  dim3(256), static_cast<double *>(s->nod.f[k]),
+                  (long long)s->nn, 0.0);
+    else
+      rc = launch(*s, "fill", fill_f32_kernel, dim3(grid_for(s->nn, 256)),
+                  dim3(256), static_cast<float *>(s->nod.f[k]),
+                  (long long)s->nn, 0.0f);
+    if (rc) return rc;
+  }
+  return launch_fill_int(*s, s->err, 1, 0);
+}
 
+// DATA momentum different from vpx*Mp (the first P2G then reads it,
+// optim_par.cpp:151-152; afterwards mom_p == vpx*Mp exactly).
+bool momentum_stored(size_t n, const double *vpx, const double *vpy,
+                     const double *Mp, const double *mom_px,
+                     const double *mom_py) {
+  for (size_t i = 0; i < n; ++i)
+    if (!(mom_px[i] == vpx[i] * Mp[i]) || !(mom_py[i] == vpy[i] * Mp[i]) ||
+        std::signbit(mom_px[i]) != std::signbit(vpx[i] * Mp[i]) ||
+        std::signbit(mom_py[i]) != std::signbit(vpy[i] * Mp[i]))
+      return true;
+  return false;
+}
+
+// Host particle arrays of n particles into buffer 0 from particle p0, then
+// the step-0 initialisation of particles [p0, p0+n) (after binning).
+int upload_and_init(Sim *s, size_t p0, size_t n, const double *x,
+                    const double *y, const double *Mp, const double *Ap,
+                    const double *vpx, const double *vpy,
+                    const double *mom_px, const double *mom_py,
+                    const double *hp, const double *Vp) {
   hipStream_t st = s->st;
-  const size_t n = (size_t)npt;
   PPtr &b = s->buf[0];
   double *stage = nullptr, *mtmp[2] = {nullptr, nullptr};
   if (n > 0) {
@@ -809,24 +849,20 @@ int create(Sim *s, int S, const amsc_params *prm, const int *np_arr,
       return fail(AMSC_ERR_NOMEM, "staging allocation failed");
     }
   }
-  auto release = [&]() {
-    (void)hipStreamSynchronize(st);
-    if (stage) (void)hipFree(stage);
-    if (mtmp[0]) (void)hipFree(mtmp[0]);
-    if (mtmp[1]) (void)hipFree(mtmp[1]);
-  };
   int rc = AMSC_OK;
   if (n > 0) {
     const double *wide_src[] = {x, y, hp};
     const int wide_dst[] = {PF_X, PF_Y, PF_HP};
     for (int k = 0; k < 3 && !rc; ++k)
-      if (hipMemcpyAsync(b.f[wide_dst[k]], wide_src[k], sizeof(double) * n,
+      if (hipMemcpyAsync(static_cast<double *>(b.f[wide_dst[k]]) + p0,
+                         wide_src[k], sizeof(double) * n,
                          hipMemcpyHostToDevice, st) != hipSuccess)
         rc = fail(AMSC_ERR_HIP, "upload failed");
     const double *nar_src[] = {vpx, vpy, Vp, Mp, Ap};
     const int nar_dst[] = {PF_VPX, PF_VPY, PF_VP, PF_MP, PF_AP};
     for (int k = 0; k < 5 && !rc; ++k) {
-      rc = upload_narrow(*s, b.f[nar_dst[k]], nar_src[k], n, stage);
+      char *dst = static_cast<char *>(b.f[nar_dst[k]]) + s->rsz * p0;
+      rc = upload_narrow(*s, dst, nar_src[k], n, stage);
       // the staging buffer is reused: wait for the narrowing kernel
       if (!rc && s->mixed && hipStreamSynchronize(st) != hipSuccess)
         rc = fail(AMSC_ERR_HIP, "upload failed");
@@ -837,64 +873,174 @@ int create(Sim *s, int S, const amsc_params *prm, const int *np_arr,
                                hipMemcpyHostToDevice, st) != hipSuccess))
       rc = fail(AMSC_ERR_HIP, "upload failed");
   }
-  if (!rc && hipMemcpyAsync(s->Z, Z, sizeof(double) * s->nn,
+  if (!rc && hipMemcpyAsync(s->poff, s->poff_h.data(), sizeof(int) * (s->S + 1),
                             hipMemcpyHostToDevice, st) != hipSuccess)
     rc = fail(AMSC_ERR_HIP, "upload failed");
-  for (int k = 0; k < NF_COUNT && !rc; ++k) {
-    if (nf_bytes(*s, k) == 8)
-      rc = launch(*s, "fill", fill_f64_kernel, dim3(grid_for(s->nn, 256)),
-                  dim3(256), static_cast<double *>(s->nod.f[k]),
-                  (long long)s->nn, 0.0);
-    else
-      rc = launch(*s, "fill", fill_f32_kernel, dim3(grid_for(s->nn, 256)),
-                  dim3(256), static_cast<float *>(s->nod.f[k]),
-                  (long long)s->nn, 0.0f);
-  }
-  // The first step's P2G uses the DATA momentum (optim_par.cpp:151-152);
-  // afterwards mom_p == vpx * Mp exactly, which P2G recomputes.
-  for (size_t i = 0; i < n && !s->mom_stored; ++i)
-    if (!(mom_px[i] == vpx[i] * Mp[i]) || !(mom_py[i] == vpy[i] * Mp[i]) ||
-        std::signbit(mom_px[i]) != std::signbit(vpx[i] * Mp[i]) ||
-        std::signbit(mom_py[i]) != std::signbit(vpy[i] * Mp[i]))
-      s->mom_stored = true;
-  if (!rc) {
-    std::vector<Ctrl> c0(S);
-    for (auto &c : c0) {
-      std::memset(&c, 0, sizeof(c));
-      c.dt = 1.0e-3;  // gpu_offload.cpp:255
-      c.active = 1;
-    }
-    if (hipMemcpyAsync(s->ctrl, c0.data(), sizeof(Ctrl) * S,
-                       hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(s->poff, s->poff_h.data(), sizeof(int) * (S + 1),
-                       hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(s->cell_off, s->coff_h.data(), sizeof(int) * (S + 1),
-                       hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      rc = fail(AMSC_ERR_HIP, "upload failed");
-  }
-  if (!rc) rc = launch_fill_int(*s, s->err, 1, 0);
+  if (!rc && hipMemcpyAsync(s->cell_off, s->coff_h.data(),
+                            sizeof(int) * (s->S + 1), hipMemcpyHostToDevice,
+                            st) != hipSuccess)
+    rc = fail(AMSC_ERR_HIP, "upload failed");
   if (!rc) rc = update_wscen(*s);
   if (!rc) rc = rebin(*s, false);
   if (!rc && n > 0) {
     const Tables T = tables(*s);
+    const dim3 g(grid_for(n, 256)), blk(256);
+    const int pb = (int)p0;
     if (s->mixed)
-      rc = launch(*s, "init", init_kernel<float>, dim3(grid_for(n, 256)),
-                  dim3(256), T, b, (const int *)s->key[s->kcur],
-                  (const double *)s->Z, (const double *)mtmp[0],
-                  (const double *)mtmp[1]);
+      rc = launch(*s, "init", init_kernel<float>, g, blk, T, b,
+                  (const int *)s->key[s->kcur], (const double *)s->Z,
+                  (const double *)mtmp[0], (const double *)mtmp[1], pb);
     else
-      rc = launch(*s, "init", init_kernel<double>, dim3(grid_for(n, 256)),
-                  dim3(256), T, b, (const int *)s->key[s->kcur],
-                  (const double *)s->Z, (const double *)mtmp[0],
-                  (const double *)mtmp[1]);
+      rc = launch(*s, "init", init_kernel<double>, g, blk, T, b,
+                  (const int *)s->key[s->kcur], (const double *)s->Z,
+                  (const double *)mtmp[0], (const double *)mtmp[1], pb);
   }
   if (!rc && hipStreamSynchronize(st) != hipSuccess)
     rc = fail(AMSC_ERR_HIP, "initialisation failed");
-  release();
+  (void)hipStreamSynchronize(st);
+  if (stage) (void)hipFree(stage);
+  if (mtmp[0]) (void)hipFree(mtmp[0]);
+  if (mtmp[1]) (void)hipFree(mtmp[1]);
+  return rc;
+}
+
+int upload_ctrl(Sim *s, int first) {
+  std::vector<Ctrl> c0(s->S - first);
+  for (auto &c : c0) {
+    std::memset(&c, 0, sizeof(c));
+    c.dt = 1.0e-3;  // gpu_offload.cpp:255
+    c.active = 1;
+  }
+  if (c0.empty()) return AMSC_OK;
+  AMSC_HIP(hipMemcpyAsync(s->ctrl + first, c0.data(), sizeof(Ctrl) * c0.size(),
+                          hipMemcpyHostToDevice, s->st));
+  AMSC_HIP(hipStreamSynchronize(s->st));
+  return AMSC_OK;
+}
+
+int create(Sim *s, int S, const amsc_params *prm, const int *np_arr,
+           const double *x, const double *y, const double *Mp,
+           const double *Ap, const double *vpx, const double *vpy,
+           const double *mom_px, const double *mom_py, const double *hp,
+           const double *Vp, const double *Z, int precision,
+           long long capacity) {
+  if (!Z) return fail(AMSC_ERR_INVALID, "NULL Z");
+  if (int rc = setup(s, S, prm, np_arr, precision, capacity)) return rc;
+  const size_t n = (size_t)s->np;
+  if (n > 0 && (!x || !y || !Mp || !Ap || !vpx || !vpy || !mom_px ||
+                !mom_py || !hp || !Vp))
+    return fail(AMSC_ERR_INVALID, "NULL particle array");
+  AMSC_HIP(hipMemcpyAsync(s->Z, Z, sizeof(double) * s->nn,
+                          hipMemcpyHostToDevice, s->st));
+  if (int rc = upload_ctrl(s, 0)) return rc;
+  s->mom_stored = n > 0 && momentum_stored(n, vpx, vpy, Mp, mom_px, mom_py);
+  const int rc = upload_and_init(s, 0, n, x, y, Mp, Ap, vpx, vpy, mom_px,
+                                 mom_py, hp, Vp);
   s->launches.clear();
   s->event_used = 0;
   return rc;
+}
+
+// New batch = scenarios keep[] of `o` (device state copied, in that order)
+// followed by nadd scenarios from host data (step-0 initialised).
+int repack(Sim *o, Sim *s, int nkeep, const int *keep, int nadd,
+           const amsc_params *params, const int *np_new, const double *x,
+           const double *y, const double *Mp, const double *Ap,
+           const double *vpx, const double *vpy, const double *mom_px,
+           const double *mom_py, const double *hp, const double *Vp,
+           const double *Z, long long capacity) {
+  if (nkeep < 0 || nadd < 0 || nkeep + nadd <= 0 || (nkeep && !keep) ||
+      (nadd && (!params || !np_new || !Z)))
+    return fail(AMSC_ERR_INVALID, "bad repack arguments");
+  if (o->broken) return fail(AMSC_ERR_INVALID, "the simulation state is invalid");
+  for (int i = 0; i < nkeep; ++i)
+    if (keep[i] < 0 || keep[i] >= o->S)
+      return fail(AMSC_ERR_INVALID, "bad scenario index");
+  // host mirrors must be exact before copying
+  AMSC_HIP(hipStreamSynchronize(o->st));
+  if (int rc = read_poff(*o)) return rc;
+  std::vector<amsc_params> prm;
+  std::vector<int> nps;
+  for (int i = 0; i < nkeep; ++i) {
+    prm.push_back(o->prm[keep[i]]);
+    nps.push_back(o->poff_h[keep[i] + 1] - o->poff_h[keep[i]]);
+  }
+  size_t nnew = 0;
+  for (int i = 0; i < nadd; ++i) {
+    prm.push_back(params[i]);
+    nps.push_back(np_new[i]);
+    if (np_new[i] < 0) return fail(AMSC_ERR_INVALID, "negative particle count");
+    nnew += np_new[i];
+  }
+  if (nnew > 0 && (!x || !y || !Mp || !Ap || !vpx || !vpy || !mom_px ||
+                   !mom_py || !hp || !Vp))
+    return fail(AMSC_ERR_INVALID, "NULL particle array");
+  if (nnew > 0 && momentum_stored(nnew, vpx, vpy, Mp, mom_px, mom_py))
+    return fail(AMSC_ERR_INVALID,
+                "scenarios added to a running batch start from momentum vpx*Mp");
+  if (int rc = setup(s, nkeep + nadd, prm.data(), nps.data(),
+                     o->mixed ? AMSC_PREC_MIXED : AMSC_PREC_FP64, capacity))
+    return rc;
+  hipStream_t st = s->st;
+  // finish the old batch's work before reading its buffers on another stream
+  AMSC_HIP(hipStreamSynchronize(o->st));
+  const PPtr &src = o->buf[o->cur];
+  PPtr &dst = s->buf[0];
+  for (int i = 0; i < nkeep; ++i) {
+    const int k = keep[i];
+    const size_t sp = o->poff_h[k], dp = s->poff_h[i], n = nps[i];
+    if (n > 0) {
+      for (int f = 0; f < PF_COUNT; ++f) {
+        const size_t e = pf_bytes(*s, f);
+        AMSC_HIP(hipMemcpyAsync(static_cast<char *>(dst.f[f]) + e * dp,
+                                static_cast<const char *>(src.f[f]) + e * sp,
+                                e * n, hipMemcpyDeviceToDevice, st));
+      }
+      AMSC_HIP(hipMemcpyAsync(dst.label + dp, src.label + sp, sizeof(int) * n,
+                              hipMemcpyDeviceToDevice, st));
+      AMSC_HIP(hipMemcpyAsync(dst.level + dp, src.level + sp, sizeof(int) * n,
+                              hipMemcpyDeviceToDevice, st));
+    }
+    const size_t sn = o->noff_h[k], dn = s->noff_h[i];
+    const size_t nn = o->noff_h[k + 1] - o->noff_h[k];
+    for (int f = 0; f < NF_COUNT; ++f) {
+      const size_t e = nf_bytes(*s, f);
+      AMSC_HIP(hipMemcpyAsync(static_cast<char *>(s->nod.f[f]) + e * dn,
+                              static_cast<const char *>(o->nod.f[f]) + e * sn,
+                              e * nn, hipMemcpyDeviceToDevice, st));
+    }
+    AMSC_HIP(hipMemcpyAsync(s->Z + dn, o->Z + sn, sizeof(double) * nn,
+                            hipMemcpyDeviceToDevice, st));
+    AMSC_HIP(hipMemcpyAsync(s->ctrl + i, o->ctrl + k, sizeof(Ctrl),
+                            hipMemcpyDeviceToDevice, st));
+    s->it_host[i] = o->it_host[k];
+    s->ended[i] = o->ended[k];
+    s->ms_counts[2 * i] = o->ms_counts[2 * k];
+    s->ms_counts[2 * i + 1] = o->ms_counts[2 * k + 1];
+  }
+  if (nadd > 0) {
+    const size_t z0 = s->noff_h[nkeep];
+    AMSC_HIP(hipMemcpyAsync(s->Z + z0, Z, sizeof(double) * (s->nn - z0),
+                            hipMemcpyHostToDevice, st));
+  }
+  if (int rc = upload_ctrl(s, nkeep)) return rc;
+  s->aux_on = o->aux_on;
+  s->diag = o->diag;
+  s->graph_on = o->graph_on;
+  s->tile_cfg = o->tile_cfg;
+  s->tile_order = o->tile_order;
+  s->part_bs = o->part_bs;
+  s->timing = o->timing;
+  s->particle_steps = o->particle_steps;
+  s->aux_valid = false;
+  s->nodal_valid = false;
+  const size_t p0 = s->poff_h[nkeep];
+  const int rc = upload_and_init(s, p0, nnew, x, y, Mp, Ap, vpx, vpy, mom_px,
+                                 mom_py, hp, Vp);
+  s->launches.clear();
+  s->event_used = 0;
+  if (rc) return rc;
+  return read_ctrl(*s);
 }
 
 int find_pf(const char *name) {
@@ -1063,10 +1209,13 @@ int avg_time(Sim &s, const char *name, double *avg) {
   return AMSC_OK;
 }
 
-// Time every P2G tile shape, then every particle workgroup size, over
-// `steps` live steps each (the results are the same whichever variant runs
-// a step) and keep the fastest of each.
-int autotune(Sim &s, int steps, int *cfg_out, int *bs_out, int *used) {
+// Time every P2G variant (tile shape x tile order), then every particle
+// workgroup size, over `steps` live steps each (the results are the same
+// whichever variant runs a step) and keep the fastest of each.
+// variant_ms (may be null): kP2GVariants P2G averages, then the particle
+// averages of the workgroup sizes.
+int autotune(Sim &s, int steps, int *cfg_out, int *bs_out, int *used,
+             double *variant_ms) {
   if (steps <= 0) return fail(AMSC_ERR_INVALID, "steps must be positive");
   const bool timing0 = s.timing;
   s.timing = true;
@@ -1079,36 +1228,67 @@ int autotune(Sim &s, int steps, int *cfg_out, int *bs_out, int *used) {
     return avg_time(s, name, avg);
   };
   double best = 1e300;
-  int bcfg = s.tile_cfg;
-  for (int c = 0; c < kTileCfgs && !s.finished; ++c) {
-    s.tile_cfg = c;
+  int bv = s.tile_cfg + kTileCfgs * s.tile_order;
+  for (int v = 0; v < kP2GVariants && !s.finished; ++v) {
+    s.tile_cfg = v % kTileCfgs;
+    s.tile_order = v / kTileCfgs;
     double a = 0.0;
     if (int rc = trial("p2g_nodal", &a)) return rc;
+    if (variant_ms) variant_ms[v] = a;
     if (a > 0.0 && a < best) {
       best = a;
-      bcfg = c;
+      bv = v;
     }
   }
-  s.tile_cfg = bcfg;
+  s.tile_cfg = bv % kTileCfgs;
+  s.tile_order = bv / kTileCfgs;
   best = 1e300;
   int bbs = s.part_bs;
-  for (int b : kPartBS) {
-    if (s.finished) break;
-    s.part_bs = b;
+  for (int i = 0; i < 2 && !s.finished; ++i) {
+    s.part_bs = kPartBS[i];
     double a = 0.0;
     if (int rc = trial("particle", &a)) return rc;
+    if (variant_ms) variant_ms[kP2GVariants + i] = a;
     if (a > 0.0 && a < best) {
       best = a;
-      bbs = b;
+      bbs = kPartBS[i];
     }
   }
   s.part_bs = bbs;
   s.timing = timing0;
   (void)reset_times(s);
-  if (cfg_out) *cfg_out = s.tile_cfg;
+  if (cfg_out) *cfg_out = bv;
   if (bs_out) *bs_out = s.part_bs;
   if (used) *used = total;
   return AMSC_OK;
+}
+
+// Average duration of `reps` back-to-back P2G launches on the current state
+// (P2G is a function of the particle state, so repeating it rewrites the
+// same nodal velocities): the bracketing events are amortised over the run.
+int time_p2g(Sim &s, int reps, double *avg_ms) {
+  if (reps <= 0 || !avg_ms) return fail(AMSC_ERR_INVALID, "bad arguments");
+  if (s.broken) return fail(AMSC_ERR_INVALID, "the simulation state is invalid");
+  const bool timing0 = s.timing;
+  s.timing = false;
+  hipEvent_t e0, e1;
+  AMSC_HIP(hipEventCreate(&e0));
+  AMSC_HIP(hipEventCreate(&e1));
+  const P2GArgs a = p2g_args(s, AUX_NONE, false);
+  int rc = AMSC_OK;
+  if (hipEventRecord(e0, s.st) != hipSuccess) rc = fail(AMSC_ERR_HIP, "event");
+  for (int i = 0; i < reps && !rc; ++i)
+    rc = s.mixed ? launch_p2g<float>(s, a) : launch_p2g<double>(s, a);
+  if (!rc && hipEventRecord(e1, s.st) != hipSuccess) rc = fail(AMSC_ERR_HIP, "event");
+  float ms = 0.f;
+  if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = fail(AMSC_ERR_HIP, "event timing");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s.timing = timing0;
+  *avg_ms = ms / reps;
+  return rc;
 }
 
 // Device bytes create() allocates for a batch (admission estimate).
@@ -1207,18 +1387,26 @@ int amsc_sim_set_graph(amsc_sim *s, int on) {
 
 int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block) {
   if (!s) return bad_handle();
-  if (p2g_shape < 0 || p2g_shape >= kTileCfgs ||
+  if (p2g_shape < 0 || p2g_shape >= kP2GVariants ||
       (particle_block != 256 && particle_block != 512))
     return fail(AMSC_ERR_INVALID, "unknown tiling");
-  s->tile_cfg = p2g_shape;
+  s->tile_cfg = p2g_shape % kTileCfgs;
+  s->tile_order = p2g_shape / kTileCfgs;
   s->part_bs = particle_block;
   return AMSC_OK;
 }
 
 int amsc_sim_autotune(amsc_sim *s, int steps_per_variant, int *p2g_shape,
-                      int *particle_block, int *steps_used) {
+                      int *particle_block, int *steps_used,
+                      double *variant_ms) {
   if (!s) return bad_handle();
-  return autotune(*s, steps_per_variant, p2g_shape, particle_block, steps_used);
+  return autotune(*s, steps_per_variant, p2g_shape, particle_block, steps_used,
+                  variant_ms);
+}
+
+int amsc_sim_time_p2g(amsc_sim *s, int reps, double *avg_ms) {
+  if (!s) return bad_handle();
+  return time_p2g(*s, reps, avg_ms);
 }
 
 int amsc_sim_kernel_times(amsc_sim *s, const char **names, double *ms,
@@ -1335,6 +1523,28 @@ int amsc_ens_info(amsc_ens *e, int *nscen, int *precision, long long *bytes,
   if (precision) *precision = e->mixed ? AMSC_PREC_MIXED : AMSC_PREC_FP64;
   if (bytes) *bytes = (long long)e->bytes;
   if (capacity) *capacity = e->cap;
+  return AMSC_OK;
+}
+
+int amsc_ens_repack(amsc_ens *e, int nkeep, const int *keep, int nadd,
+                    const amsc_params *params, const int *np, const double *x,
+                    const double *y, const double *Mp, const double *Ap,
+                    const double *vpx, const double *vpy,
+                    const double *mom_px, const double *mom_py,
+                    const double *hp, const double *Vp, const double *Z,
+                    long long capacity, amsc_ens **out) {
+  if (!e || !out) return fail(AMSC_ERR_INVALID, "NULL handle");
+  *out = nullptr;
+  amsc_sim *s = new amsc_sim();
+  const int rc = repack(e, s, nkeep, keep, nadd, params, np, x, y, Mp, Ap,
+                        vpx, vpy, mom_px, mom_py, hp, Vp, Z, capacity);
+  if (rc) {
+    const std::string msg = last_error_cstr();
+    sim_free(s);
+    return fail(rc, msg);
+  }
+  sim_free(e);
+  *out = s;
   return AMSC_OK;
 }
 

@@ -206,11 +206,14 @@ __global__ void __launch_bounds__(64)
 // ------------------------------------------------------------- step 0
 // optim_par.cpp:235-264 (cells from the initial binning): Zp, hpZ, H,
 // hydrostatic F, Fric = Ap*Fb with Fb = 0.
+// Particles [p_begin, poff[S]) (all of them at creation, the scenarios just
+// added when a served batch is repacked); mom_*_in are indexed from p_begin.
 template <typename R>
 __global__ void init_kernel(Tables T, PPtr p, const int *key,
                             const double *__restrict__ Z,
-                            const double *mom_x_in, const double *mom_y_in) {
-  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+                            const double *mom_x_in, const double *mom_y_in,
+                            int p_begin) {
+  const int ip = p_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= T.poff[T.S]) return;
   const int sid = scen_of(T, ip);
   const Scen &S = T.scen[sid];
@@ -240,8 +243,8 @@ __global__ void init_kernel(Tables T, PPtr p, const int *key,
                              PF_APY,  PF_DZX,  PF_DZY,  PF_FPX,  PF_FPY};
 #pragma unroll
   for (int k = 0; k < 10; ++k) fld<R>(p, zero_fields[k])[ip] = R(0);
-  fld<R>(p, PF_MOMX)[ip] = (R)mom_x_in[ip];
-  fld<R>(p, PF_MOMY)[ip] = (R)mom_y_in[ip];
+  fld<R>(p, PF_MOMX)[ip] = (R)mom_x_in[ip - p_begin];
+  fld<R>(p, PF_MOMY)[ip] = (R)mom_y_in[ip - p_begin];
   p.label[ip] = ip - T.poff[sid];
   p.level[ip] = 0;
 }
@@ -337,28 +340,32 @@ __device__ __forceinline__ int lds_slot(int i) { return i + (i >> 4); }
 // the oracle's bit for bit.  Workgroups are mapped to tiles so that each of
 // the 8 XCDs (workgroup i runs on XCD i mod 8) gets a contiguous range of
 // tiles, whose shared halo cells then hit the same L2.
-template <typename R, int TC, int TR>
+template <typename R, int TC, int TR, bool XCD>
 __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   constexpr int NT = TC * TR, CH = NT, CHP = CH + CH / 16, SEG = TC + 1;
   static_assert(TR + 1 < 256 && SEG < 256, "owner map packs (s, j) in 16 bits");
   const int nb = gridDim.x, b = blockIdx.x;
-  const int xcd = b & 7, q8 = nb >> 3, rem = nb & 7;
-  const int tile = xcd * q8 + (xcd < rem ? xcd : rem) + (b >> 3);
+  int tile = b;
+  if (XCD) {
+    const int xcd = b & 7, q8 = nb >> 3, rem = nb & 7;
+    tile = xcd * q8 + (xcd < rem ? xcd : rem) + (b >> 3);
+  }
   const int sid = a.tscen[tile];
   const Scen &sc = a.T.scen[sid];
   Ctrl *ctrl = &a.T.ctrl[sid];
   if (!ctrl->active) return;
-  const int nrows = sc.P.nrows, ncols = sc.P.ncols;
-  const double hx = sc.P.hx, hy = sc.P.hy;
+  const int nrows = sc.P.nrows, ncols = sc.P.ncols, bc = sc.P.bc;
+  const double hx = sc.P.hx, hy = sc.P.hy, grav = sc.P.g, rho = sc.P.rho;
+  const int coff = sc.coff, noff = sc.noff, toff = sc.toff, ntx = sc.ntx;
   const Recip rc = sc.rc;
   const RecipT<R> rcr = recip_of<R>(sc);
-  const int lt = tile - sc.toff;
-  const int c0 = (lt % sc.ntx) * TC, r0 = (lt / sc.ntx) * TR;
+  const int lt = tile - toff;
+  const int c0 = (lt % ntx) * TC, r0 = (lt / ntx) * TR;
   const int t = threadIdx.x;
   const int lr = t % TR, lc = t / TR;
   const int c = c0 + lc, r = r0 + lr;
   const bool node_ok = c <= ncols && r <= nrows;
-  const int *cstart = a.cstart + sc.coff;
+  const int *cstart = a.cstart + coff;
 
   __shared__ int s_cs[SEG][TR + 2];  // cstart, cell rows rl..rh+1
   __shared__ int s_sego[SEG + 1];    // tile-list offset per column
@@ -422,7 +429,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     for (int i = t; i < (TC + 2) * (TR + 2); i += NT) {
       const int zc = c0 - 1 + i / (TR + 2), zr = r0 - 1 + i % (TR + 2);
       s_Z[i] = (zc >= 0 && zc <= ncols && zr >= 0 && zr <= nrows)
-                   ? a.Z[sc.noff + zr + zc * (nrows + 1)]
+                   ? a.Z[noff + zr + zc * (nrows + 1)]
                    : 0.0;
     }
     // this node's four cells as tile-list ranges, ascending cell id:
@@ -469,8 +476,8 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
         // optim_par.cpp:450-453
         const double wb = W.hp > 1e-10 ? 1.0 - (W.H / W.hp) : 1.0;
         const double Vpd = W.Vp;
-        const double Fpx = -sc.P.g * Vpd * sc.P.rho * dzx * wb;
-        const double Fpy = -sc.P.g * Vpd * sc.P.rho * dzy * wb;
+        const double Fpx = -grav * Vpd * rho * dzx * wb;
+        const double Fpy = -grav * Vpd * rho * dzy * wb;
         // mom_p = vpx * Mp (optim_par.cpp:600-601), DATA momentum at step 1
         const double Mpd = W.Mp;
         const double mx = a.mom_stored ? (double)W.m1 : (double)W.m1 * Mpd;
@@ -542,7 +549,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   }
 
   if (node_ok && (L > 0 || a.aux >= AUX_FULL)) {
-    const int n = sc.noff + r + c * (nrows + 1);
+    const int n = noff + r + c * (nrows + 1);
     const double dt = ctrl->dt;
     // force balance and nodal velocities, optim_par.cpp:467-497 (D1)
     const R Fex = FPx + Frx, Fey = FPy + Fry;
@@ -555,7 +562,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     double vxL = dt * ax + vx, vyL = dt * ay + vy;
     // boundary conditions, optim_par.cpp:503-529 (nodes of the first/last
     // cell column: x components; first/last cell row: y components)
-    if (sc.P.bc) {
+    if (bc) {
       if (c == 0 || c == 1 || c == ncols - 1 || c == ncols) {
         ax = 0.0; vx = 0.0; vxL = 0.0; mx = 0.0;
       }

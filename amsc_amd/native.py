@@ -54,6 +54,19 @@ class AmscParams(C.Structure):
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class _Nullable:
+    """An ndpointer argument that also accepts None (NULL)."""
+
+    def __init__(self, base):
+        self.base = base
+
+    def from_param(self, obj):
+        return None if obj is None else self.base.from_param(obj)
+
+
+_dpn, _ipn = _Nullable(_dp), _Nullable(_ip)
 _vp = C.c_void_p
 
 # The 18 step kernels: argument kinds in gpu_kernels.h order.
@@ -131,7 +144,12 @@ def lib():
     L.amsc_sim_set_graph.argtypes = [_vp, C.c_int]
     L.amsc_sim_set_tiling.argtypes = [_vp, C.c_int, C.c_int]
     L.amsc_sim_autotune.argtypes = [_vp, C.c_int, C.POINTER(C.c_int),
-                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)]
+    L.amsc_sim_time_p2g.argtypes = [_vp, C.c_int, C.POINTER(C.c_double)]
+    L.amsc_ens_repack.argtypes = [_vp, C.c_int, _ipn, C.c_int,
+                                  C.POINTER(AmscParams), _ipn] + [_dpn] * 11 + \
+        [C.c_longlong, C.POINTER(_vp)]
     L.amsc_ens_create.argtypes = [C.c_int, C.POINTER(AmscParams), _ip] + \
         [_dp] * 11 + [C.c_int, C.c_longlong, C.POINTER(_vp)]
     L.amsc_ens_status.argtypes = [_vp, _ip, _dp, _dp, _ip, _ip]

@@ -1,0 +1,231 @@
+"""Served hazard-ensemble requests on one GPU.
+
+Requests (scenarios of uneven grid and particle counts, each run to its own
+simulated end time) arrive as a Poisson process.  Two forms serve them:
+
+* batched: every admitted request is a scenario of one device-resident
+  `Ensemble`; the batch advances `chunk` steps at a time; finished scenarios
+  leave and arrived ones join through a device-side repack (the state of a
+  running scenario never leaves the GPU).  Admission keeps the device bytes
+  of the old and the new batch together (both exist during a repack) within
+  the HBM budget;
+* sequential: one `Simulation` per request, first come first served (the
+  single-scenario driver loop).
+
+Both report time-to-solution (arrival to completion, wall clock) per request,
+its p50/p99, and completed scenarios per second over the run.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from .data import params_from_data, particles_from_data
+
+PARTICLE_KEYS = ("x", "y", "Mp", "Ap", "vpx", "vpy", "mom_px", "mom_py", "hp", "Vp")
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds after the start of the run
+    prm: dict
+    particles: dict
+    Z: np.ndarray
+    admitted: float = math.nan
+    done: float = math.nan
+    steps: int = 0
+    result: dict | None = None
+    cur_np: int = -1  # particle count while running (merge/split moves it)
+
+    @property
+    def np(self) -> int:
+        return len(self.particles["x"])
+
+    @property
+    def live_np(self) -> int:
+        return self.cur_np if self.cur_np >= 0 else self.np
+
+
+def poisson_arrivals(rate: float, n: int, seed: int) -> np.ndarray:
+    """Arrival times of n requests of a Poisson process of `rate` per s."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, n))
+
+
+def make_requests(datas, arrivals, max_steps: int = 0):
+    out = []
+    for i, (d, t) in enumerate(zip(datas, arrivals)):
+        prm = params_from_data(d, max_steps=max_steps)
+        out.append(Request(i, float(t), prm, particles_from_data(d), np.asarray(d["Z"])))
+    return out
+
+
+def _pcts(tts):
+    a = np.asarray(tts, np.float64)
+    if a.size == 0:
+        return {"p50_s": None, "p99_s": None, "mean_s": None}
+    return {"p50_s": float(np.percentile(a, 50)), "p99_s": float(np.percentile(a, 99)),
+            "mean_s": float(a.mean())}
+
+
+def _cat(reqs, key):
+    return np.ascontiguousarray(np.concatenate(
+        [np.asarray(r.particles[key], np.float64) for r in reqs]))
+
+
+class BatchedServer:
+    """Batched form over amsc_ens_* (see module docstring)."""
+
+    def __init__(self, precision: str = "mixed", budget_bytes: int | None = None,
+                 chunk: int = 10, graph: bool = True, capacity_factor: float = 4.0,
+                 collect=()):
+        self.precision = precision
+        self.budget = budget_bytes
+        self.chunk = chunk
+        self.graph = graph
+        self.capacity_factor = capacity_factor
+        self.collect = tuple(collect)  # fields kept from each finished request
+
+    def _capacity(self, reqs):
+        # merge/split may split particles twice (ms_min_level = -2): room for
+        # 4x the present count by default
+        return int(self.capacity_factor * max(1, sum(r.live_np for r in reqs)))
+
+    def _bytes(self, reqs):
+        from .simulation import Ensemble
+        if not reqs:
+            return 0
+        return Ensemble.bytes_estimate([r.prm for r in reqs], [r.live_np for r in reqs],
+                                       self.precision, self._capacity(reqs))
+
+    def run(self, requests, clock=time.perf_counter, sleep=time.sleep, log=None):
+        import ctypes as C
+        from . import native
+        from .simulation import Ensemble, PRECISION
+        L = native.lib()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active: list[Request] = []
+        ens = None
+        t0 = clock()
+        repacks = 0
+        while pending or active:
+            now = clock() - t0
+            # admission, first come first served, within the budget
+            joining = []
+            while pending and pending[0].arrival <= now:
+                cand = active + joining + [pending[0]]
+                need = self._bytes(cand) + (self._bytes(active) if ens else 0)
+                if self.budget is not None and need > self.budget and (active or joining):
+                    break
+                r = pending.pop(0)
+                r.admitted = now
+                joining.append(r)
+            if joining:
+                keep = np.arange(len(active), dtype=np.int32)
+                newreq = active + joining
+                cap = self._capacity(newreq)
+                P = (native.AmscParams * len(joining))(
+                    *[native.AmscParams.from_dict(r.prm) for r in joining])
+                nps = np.array([r.np for r in joining], np.int32)
+                arrs = [_cat(joining, k) for k in PARTICLE_KEYS]
+                Z = np.ascontiguousarray(np.concatenate(
+                    [np.asarray(r.Z, np.float64).ravel() for r in joining]))
+                if ens is None:
+                    ens = Ensemble.__new__(Ensemble)
+                    h = C.c_void_p()
+                    native.check(L.amsc_ens_create(len(joining), P, nps, *arrs, Z,
+                                                   PRECISION[self.precision], cap,
+                                                   C.byref(h)))
+                    ens._h = h
+                    if self.graph:
+                        ens.set_graph(True)
+                else:
+                    h = C.c_void_p()
+                    native.check(L.amsc_ens_repack(ens._h, len(keep), keep, len(joining),
+                                                   P, nps, *arrs, Z, cap, C.byref(h)))
+                    ens._h = h
+                    repacks += 1
+                active = newreq
+                ens.nscen = len(active)
+                ens.prms = [r.prm for r in active]
+            if not active:
+                if pending:
+                    sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.05)))
+                continue
+            done = C.c_int()
+            native.check(L.amsc_sim_step(ens._h, self.chunk, C.byref(done)))
+            st = ens.status()
+            now = clock() - t0
+            still = []
+            for i, r in enumerate(active):
+                r.steps = int(st["it"][i])
+                r.cur_np = int(st["np"][i])
+                if st["active"][i]:
+                    still.append(i)
+                else:
+                    r.done = now
+            if len(still) != len(active):
+                keep_set = set(still)
+                finished = [r for i, r in enumerate(active) if i not in keep_set]
+                if self.collect:
+                    for i, r in enumerate(active):
+                        if i not in keep_set:
+                            r.result = {f: ens.get(i, f) for f in self.collect}
+                if log:
+                    for r in finished:
+                        log(f"[serve] request {r.rid} done: {r.steps} steps, "
+                            f"{r.done - r.arrival:.3f} s")
+                if still:
+                    keep = np.array(still, np.int32)
+                    h = C.c_void_p()
+                    native.check(L.amsc_ens_repack(
+                        ens._h, len(keep), keep, 0, None, None, *([None] * 11),
+                        self._capacity([active[i] for i in still]), C.byref(h)))
+                    ens._h = h
+                    repacks += 1
+                    active = [active[i] for i in still]
+                    ens.nscen = len(active)
+                    ens.prms = [r.prm for r in active]
+                else:
+                    ens.close()
+                    ens = None
+                    active = []
+        wall = clock() - t0
+        if ens is not None:
+            ens.close()
+        tts = [r.done - r.arrival for r in requests]
+        return {"form": f"batched-{self.precision}", "requests": len(requests),
+                "wall_s": wall, "scen_per_s": len(requests) / wall,
+                "repacks": repacks, **_pcts(tts)}
+
+
+class SequentialServer:
+    """One `Simulation` (FP64) per request, first come first served."""
+
+    def __init__(self, chunk: int = 100):
+        self.chunk = chunk
+
+    def run(self, requests, clock=time.perf_counter, sleep=time.sleep, log=None):
+        from .simulation import Simulation
+        pending = sorted(requests, key=lambda r: r.arrival)
+        t0 = clock()
+        for r in pending:
+            now = clock() - t0
+            if r.arrival > now:
+                sleep(r.arrival - now)
+            r.admitted = clock() - t0
+            s = Simulation(r.prm, r.particles, r.Z, capacity=4 * max(r.np, 1))
+            r.steps = s.run(self.chunk)
+            r.done = clock() - t0
+            s.close()
+            if log:
+                log(f"[serve] request {r.rid} done: {r.steps} steps, "
+                    f"{r.done - r.arrival:.3f} s")
+        wall = clock() - t0
+        tts = [r.done - r.arrival for r in requests]
+        return {"form": "sequential-fp64", "requests": len(requests), "wall_s": wall,
+                "scen_per_s": len(requests) / wall, **_pcts(tts)}

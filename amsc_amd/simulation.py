@@ -156,14 +156,27 @@ class Simulation:
         native.check(native.lib().amsc_sim_set_tiling(self._h, int(p2g_shape),
                                                       int(particle_block)))
 
-    def autotune(self, steps_per_variant: int = 2):
-        """Pick the fastest P2G tile shape and particle workgroup size on live
-        steps; returns (shape, block, steps used)."""
+    def autotune(self, steps_per_variant: int = 2, report: dict | None = None):
+        """Pick the fastest P2G variant and particle workgroup size on live
+        steps; returns (variant, block, steps used).  `report` (optional)
+        receives the average ms of every variant."""
         a, b, u = C.c_int(), C.c_int(), C.c_int()
+        ms = (C.c_double * 8)()
         native.check(native.lib().amsc_sim_autotune(self._h, int(steps_per_variant),
                                                     C.byref(a), C.byref(b),
-                                                    C.byref(u)))
+                                                    C.byref(u), ms))
+        if report is not None:
+            report.update({f"p2g_{v}": round(ms[v], 5) for v in range(6)})
+            report.update({"particle_256": round(ms[6], 5),
+                           "particle_512": round(ms[7], 5)})
         return a.value, b.value, u.value
+
+    def time_p2g(self, reps: int = 20) -> float:
+        """Average ms of `reps` back-to-back P2G launches on the current
+        state."""
+        out = C.c_double()
+        native.check(native.lib().amsc_sim_time_p2g(self._h, int(reps), C.byref(out)))
+        return out.value
 
 
 class Ensemble(Simulation):

@@ -1,20 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the depth-averaged MPM time step on one MI355X.
 
-Workload (default): the slope landslide of amsc_amd.scenarios
-.slope_landslide(): 1000x1000 cells (hx = 1 m), Z = 0.3 x + 8 seeded Gaussian
-bumps, an elliptical release of 2.0M particles at 4 per cell, Voellmy
-(phi = 15 deg, xi = 500) + Bingham (mu = 0.5, tauy = 200), rho = 2000,
-adaptive dt with CFL = 0.4, merge/split every 10 steps; FP64 throughout.
-A "step" is one full time step (dt, merge/split when due, P2G + nodal
-update, G2P/advection/height/rheology, re-binning) with all state resident
-in HBM.
+Headline workload (default): the slope landslide of
+amsc_amd.scenarios.slope_landslide(): 1000x1000 cells (hx = 1 m),
+Z = 0.3 x + 8 seeded Gaussian bumps, an elliptical release of 2.0M particles
+at 4 per cell, Voellmy (phi = 15 deg, xi = 500) + Bingham (mu = 0.5,
+tauy = 200), rho = 2000, adaptive dt with CFL = 0.4, merge/split every 10
+steps; FP64 throughout.  A "step" is one full time step (dt, merge/split
+when due, P2G + nodal update, G2P/advection/height/rheology, re-binning)
+with all state resident in HBM.
 
-Prints one JSON line: particle-steps/s over the timed steps, the HBM
-roofline of the dominant kernel (algorithmic bytes per launch over its
-average HIP-event duration, in a second pass with per-kernel events), and
-the FP64 CPU oracle timed on a bounded sample of the same workload on the
-host (1 thread).
+The JSON line reports:
+  value          FP64 particle-steps/s over the timed steps (plain form);
+  roofline       HBM roofline of the dominant kernel (P2G): algorithmic bytes
+                 per launch over its average duration (back-to-back launches
+                 on the live state, bracketed by HIP events on the solver's
+                 stream), traffic from the committed rocprofv3 counter runs;
+  mixed          the same workload in mixed FP32/FP64, with its error
+                 against the FP64 run;
+  ensemble       256 hazard scenarios (side ~ logU[64,1024] cells, particles
+                 ~ logU[1e4,1e6] at 4 per cell, seeded topography and
+                 rheology) advanced 100 steps each: batched mixed (one
+                 launch per phase, HIP graphs, autotuned tiles) vs 256
+                 sequential plain FP64 runs, in scenarios/s, with the
+                 per-field error of mixed against FP64;
+  served         (--served) Poisson arrivals at 5, 20 and 80 requests/s,
+                 p50/p99 time-to-solution and scenarios/s, batched mixed vs
+                 sequential FP64;
+  cpu_baseline   the FP64 CPU oracle on a bounded sample of the headline
+                 workload (1 thread).
 
     python bench.py [--steps K] [--warmup W] [--config slope2m|large4m|dambreak]
 """
@@ -27,11 +41,14 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s (1 GPU, FP64 plain vs mixed); ensemble scen/s + p50/p99"
 HBM_PEAK_GBS = 8000.0
+HBM_BYTES = 288 * 2**30
 
 # Algorithmic bytes (FP64): fields each kernel must read or write once.
 #   p2g_nodal: reads x y Mp vpx vpy Vp hp H Fric_px Fric_py F11 F12 F22 per
@@ -43,6 +60,9 @@ HBM_PEAK_GBS = 8000.0
 #              active node.
 ALGO = {"p2g_nodal": (104, 24), "particle": (120, 16)}
 STEP_ALGO = (184, 120)
+# Mixed layout: x y hp H FP64, the other 9 state fields FP32 (read 68 B,
+# write 52 B per particle); nodal vL FP32.
+STEP_ALGO_MIXED = (120, 104)
 
 
 def log(*a):
@@ -87,15 +107,183 @@ def cpu_baseline(d, steps, warm=2):
     return ps / dt, dt, o.np
 
 
+def gridded_depth(x, y, Vp, prm):
+    """Depth per cell: particle volume in the cell over the cell area."""
+    nr, nc = prm["nrows"], prm["ncols"]
+    c = np.clip(np.floor(x / prm["hx"]).astype(np.int64), 0, nc - 1)
+    r = np.clip(np.floor(y / prm["hy"]).astype(np.int64), 0, nr - 1)
+    return np.bincount(r + c * nr, weights=Vp, minlength=nr * nc) / (prm["hx"] * prm["hy"])
+
+
+def field_errors(a, b, prm):
+    """Error of run a against run b (both Simulation-like, scenario 0):
+    gridded depth relative to the maximum depth; per particle depth and
+    velocity when the particle sets match."""
+    ga = gridded_depth(a["x"], a["y"], a["Vp"], prm)
+    gb = gridded_depth(b["x"], b["y"], b["Vp"], prm)
+    out = {"grid_depth_max": float(np.abs(ga - gb).max() / max(gb.max(), 1e-30)),
+           "grid_depth_median_wet": float(np.median(np.abs(ga - gb)[gb > 0]) /
+                                          max(gb.max(), 1e-30)) if (gb > 0).any() else 0.0}
+    if len(a["hp"]) == len(b["hp"]):
+        hmax = max(np.abs(b["hp"]).max(), 1e-30)
+        cw = np.sqrt(9.81 * hmax)
+        dh = np.abs(a["hp"] - b["hp"]) / hmax
+        dv = np.hypot(a["vpx"] - b["vpx"], a["vpy"] - b["vpy"]) / cw
+        out.update({"hp_max": float(dh.max()), "hp_median": float(np.median(dh)),
+                    "v_max_over_sqrt_gh": float(dv.max()),
+                    "v_median_over_sqrt_gh": float(np.median(dv))})
+    return out
+
+
+def grab(sim, k=None):
+    g = (lambda f: sim.get(f)) if k is None else (lambda f: sim.get(k, f))
+    return {f: g(f) for f in ("x", "y", "Vp", "hp", "vpx", "vpy")}
+
+
+def run_headline(args, d, precision):
+    """Autotune, time `steps` steps, then the per-kernel pass."""
+    from amsc_amd.simulation import Ensemble, Simulation
+    np0 = len(d["x"])
+    total = args.warmup + args.steps + args.roofline_steps + 40
+    from amsc_amd.data import params_from_data, particles_from_data
+    prm = params_from_data(d, max_steps=total)
+    if precision == "fp64":
+        sim = Simulation(prm, particles_from_data(d), d["Z"], capacity=3 * np0)
+    else:
+        sim = Ensemble([(prm, particles_from_data(d), d["Z"])], "mixed", capacity=3 * np0)
+    tune = {}
+    variant, block, used = sim.autotune(2, tune)
+    rest = max(args.warmup - used, 0)
+    sim.step(rest)
+    log(f"[bench] {precision}: autotune -> P2G variant {variant}, particle block "
+        f"{block} ({used} steps), warmup {rest} more steps")
+    ps0 = sim.particle_steps
+    t0 = time.perf_counter()
+    done = sim.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    ps = sim.particle_steps - ps0
+    res = {"value": ps / elapsed, "ms_per_step": 1e3 * elapsed / max(done, 1),
+           "steps": done, "variant": variant, "block": block, "tune_ms": tune}
+    log(f"[bench] {precision}: {res['ms_per_step']:.4f} ms/step, "
+        f"{res['value']:.4g} particle-steps/s")
+    # dominant kernel: back-to-back P2G launches on the live state
+    res["p2g_ms"] = sim.time_p2g(args.p2g_reps)
+    res["n_act"] = sim.active_nodes() if precision == "fp64" else None
+    res["np"] = sim.num_particles
+    # per-kernel pass (dispatch-attached events)
+    sim.set_timing(True)
+    sim.reset_kernel_times()
+    sim.step(args.roofline_steps)
+    kt = sim.kernel_times()
+    sim.set_timing(False)
+    res["kernels"] = {k: round(v[0] / max(v[1], 1), 5) for k, v in kt.items()}
+    return sim, res
+
+
+def ensemble_bench(args):
+    from amsc_amd import scenarios
+    from amsc_amd.data import params_from_data, particles_from_data
+    from amsc_amd.simulation import Ensemble, Simulation
+    n, K = args.ens_scenarios, args.ens_steps
+    t0 = time.perf_counter()
+    cases = []
+    for i in range(n):
+        d = scenarios.ensemble_scenario(1000 + i)
+        prm = params_from_data(d, max_steps=K)
+        cases.append((prm, particles_from_data(d), d["Z"]))
+    npart = sum(len(c[1]["x"]) for c in cases)
+    log(f"[bench] ensemble: {n} scenarios, {npart} particles, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    out = {"scenarios": n, "steps_each": K, "particles": npart}
+
+    def batched(precision):
+        e = Ensemble(cases, precision, capacity=3 * npart)
+        e.set_graph(True)
+        t = time.perf_counter()
+        e.run(K)
+        wall = time.perf_counter() - t
+        ps = e.particle_steps
+        st = e.status()
+        assert (st["it"] == K).all(), st["it"]
+        return e, wall, ps
+
+    e, wall, ps = batched("mixed")
+    out["mixed_batched"] = {"scen_per_s": n / wall, "wall_s": wall,
+                            "particle_steps_per_s": ps / wall,
+                            "p50_s": wall, "p99_s": wall}
+    log(f"[bench] ensemble mixed batched: {wall:.3f} s, {n / wall:.1f} scen/s")
+    mixed_state = [grab(e, k) for k in range(n)]
+    e.close()
+    e, wall64, ps64 = batched("fp64")
+    out["fp64_batched"] = {"scen_per_s": n / wall64, "wall_s": wall64,
+                           "particle_steps_per_s": ps64 / wall64}
+    log(f"[bench] ensemble fp64 batched: {wall64:.3f} s, {n / wall64:.1f} scen/s")
+    e.close()
+    # sequential plain FP64 (creation not timed)
+    seq, tts, errs = 0.0, [], []
+    for k, c in enumerate(cases):
+        s = Simulation(*c, capacity=3 * len(c[1]["x"]))
+        t = time.perf_counter()
+        s.run(K + 1)
+        dt = time.perf_counter() - t
+        seq += dt
+        tts.append(seq)
+        errs.append(field_errors(mixed_state[k], grab(s), c[0]))
+        s.close()
+    out["fp64_sequential"] = {"scen_per_s": n / seq, "wall_s": seq,
+                              "p50_s": float(np.percentile(tts, 50)),
+                              "p99_s": float(np.percentile(tts, 99))}
+    out["mixed_batched_vs_fp64_sequential"] = (n / wall) / (n / seq)
+    log(f"[bench] ensemble fp64 sequential: {seq:.3f} s, {n / seq:.1f} scen/s; "
+        f"ratio {out['mixed_batched_vs_fp64_sequential']:.2f}")
+    keys = sorted({k for e_ in errs for k in e_})
+    out["mixed_error_vs_fp64"] = {
+        k: {"max": float(np.max([e_[k] for e_ in errs if k in e_])),
+            "median": float(np.median([e_[k] for e_ in errs if k in e_])),
+            "scenarios": int(sum(k in e_ for e_ in errs))} for k in keys}
+    return out
+
+
+def served_bench(args):
+    from amsc_amd import scenarios, serve
+    out = {}
+    for rate in (5.0, 20.0, 80.0):
+        datas = [scenarios.ensemble_scenario(5000 + i, np_range=(1e4, 4e6))
+                 for i in range(args.served_requests)]
+        arr = serve.poisson_arrivals(rate, len(datas), seed=int(rate))
+        res = {}
+        for form in ("mixed", "fp64"):
+            reqs = serve.make_requests(datas, arr, max_steps=args.served_max_steps)
+            if form == "mixed":
+                srv = serve.BatchedServer("mixed", budget_bytes=int(0.8 * HBM_BYTES))
+            else:
+                srv = serve.SequentialServer()
+            r = srv.run(reqs)
+            r["steps_mean"] = float(np.mean([q.steps for q in reqs]))
+            res[form] = r
+            log(f"[bench] served {rate}/s {form}: {r['scen_per_s']:.2f} scen/s, "
+                f"p50 {r['p50_s']:.3f} s, p99 {r['p99_s']:.3f} s")
+        out[f"lambda_{int(rate)}"] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="slope2m")
-    ap.add_argument("--roofline-steps", type=int, default=50)
+    ap.add_argument("--roofline-steps", type=int, default=30)
+    ap.add_argument("--p2g-reps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=14)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-scenarios", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=100)
+    ap.add_argument("--served", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=40)
+    ap.add_argument("--served-max-steps", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/"
@@ -119,54 +307,45 @@ def main():
               "see DESIGN.md for how it would shard.", file=sys.stderr)
         sys.exit(2)
 
-    from amsc_amd.simulation import Simulation
-
     d, desc = workload(args.config)
     np0 = len(d["x"])
-    total = args.warmup + args.steps + args.roofline_steps + 5
     log(f"[bench] {args.config}: {np0} particles, grid {d['Nex']}x{d['Ney']}")
-    sim = Simulation.from_data(d, capacity=3 * np0, max_steps=total)
-    sim.step(args.warmup)
-    log(f"[bench] warmup {args.warmup} steps done")
-
-    ps0 = sim.particle_steps
-    t0 = time.perf_counter()
-    done = sim.step(args.steps)
-    t1 = time.perf_counter()
-    ps = sim.particle_steps - ps0
-    elapsed = t1 - t0
-    value = ps / elapsed
-    ms_per_step = 1e3 * elapsed / max(done, 1)
-    log(f"[bench] timed {done} steps: {ms_per_step:.4f} ms/step, {value:.4g} particle-steps/s")
-
-    # second pass: per-kernel HIP events on the solver's stream
-    n_act = sim.active_nodes()
-    npart = sim.num_particles
-    sim.set_timing(True)
-    sim.reset_kernel_times()
-    done_r = sim.step(args.roofline_steps)
-    kt = sim.kernel_times()
-    sim.set_timing(False)
-    n_act2 = sim.active_nodes()
-    npart2 = sim.num_particles
-    n_act_m = 0.5 * (n_act + n_act2)
-    np_m = 0.5 * (npart + npart2)
-    dom = max(ALGO, key=lambda k: kt.get(k, (0.0, 1))[0])
-    tot_ms, cnt = kt[dom]
-    avg_ms = tot_ms / max(cnt, 1)
-    bpp, bpn = ALGO[dom]
-    algo_bytes = bpp * np_m + bpn * n_act_m
-    achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
-    step_bytes = STEP_ALGO[0] * np_m + STEP_ALGO[1] * n_act_m
-    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
-    kernels = {k: round(v[0] / max(v[1], 1), 5) for k, v in kt.items()}
-    log(f"[bench] kernels (avg ms): {kernels}")
+    sim, r64 = run_headline(args, d, "fp64")
+    n_act = r64["n_act"]
+    np_m = r64["np"]
+    bpp, bpn = ALGO["p2g_nodal"]
+    algo_bytes = bpp * np_m + bpn * n_act
+    achieved = algo_bytes / (r64["p2g_ms"] * 1e-3) / 1e9
+    step_bytes = STEP_ALGO[0] * np_m + STEP_ALGO[1] * n_act
+    step_gbs = step_bytes / (r64["ms_per_step"] * 1e-3) / 1e9
+    state64 = grab(sim)
+    sim.close()
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.config}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_p2g_nodal_{args.config}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
+
+    mixed = None
+    if not args.no_mixed:
+        simm, rm = run_headline(args, d, "mixed")
+        mb = STEP_ALGO_MIXED[0] * rm["np"] + STEP_ALGO_MIXED[1] * n_act
+        mixed = {"value": round(rm["value"], 1), "unit": "particle-steps/s",
+                 "ms_per_step": round(rm["ms_per_step"], 5),
+                 "vs_fp64": round(rm["value"] / r64["value"], 3),
+                 "p2g_ms": round(rm["p2g_ms"], 5),
+                 "step_roofline_frac": round(mb / (rm["ms_per_step"] * 1e-3) / 1e9
+                                             / HBM_PEAK_GBS, 4),
+                 "tiling": {"p2g_variant": rm["variant"], "particle_block": rm["block"]},
+                 "kernels_avg_ms": rm["kernels"]}
+        from amsc_amd.data import params_from_data
+        mixed["error_vs_fp64_same_steps"] = field_errors(grab(simm, 0), state64,
+                                                          params_from_data(d))
+        simm.close()
+
+    ens = None if args.no_ensemble else ensemble_bench(args)
+    served = served_bench(args) if args.served else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -181,28 +360,33 @@ def main():
         log(f"[bench] CPU: {v:.4g} particle-steps/s")
 
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s",
-        "n_gpus": 1, "steps": done, "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+        "metric": METRIC, "value": round(r64["value"], 1), "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": r64["steps"], "warmup": args.warmup,
+        "ms_per_step": round(r64["ms_per_step"], 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "config": args.config, "particles": np0,
                    "grid": [int(d["Nex"]), int(d["Ney"])], "precision": "fp64",
-                   "active_nodes": int(n_act_m)},
-        "roofline": {"bound": "hbm", "kernel": dom,
+                   "active_nodes": int(n_act),
+                   "tiling": {"p2g_variant": r64["variant"],
+                              "particle_block": r64["block"]}},
+        "roofline": {"bound": "hbm", "kernel": "p2g_nodal",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "avg_launch_ms": round(avg_ms, 5),
+                     "avg_launch_ms": round(r64["p2g_ms"], 5),
                      "algorithmic_bytes_per_launch": int(algo_bytes)},
         "step_roofline": {"algorithmic_bytes_per_step": int(step_bytes),
                           "achieved": round(step_gbs, 1),
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
-        "kernels_avg_ms": kernels,
+        "kernels_avg_ms": r64["kernels"],
+        "autotune_ms": r64["tune_ms"],
+        "mixed": mixed,
+        "ensemble": ens,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    sim.close()
 
 
 if __name__ == "__main__":

@@ -284,14 +284,23 @@ int amsc_sim_ms_counts(amsc_sim *s, int out[2]);
  * extents then follow the particle capacity; results are unchanged).
  * Ignored while per-kernel timing is on. */
 int amsc_sim_set_graph(amsc_sim *s, int on);
-/* P2G tile shape (0: 16x32 nodes, 1: 16x16, 2: 8x32) and particle-kernel
- * workgroup size (256 or 512); results do not depend on either. */
+/* P2G variant v in 0..5: tile shape v % 3 (0: 16x32 nodes, 1: 16x16,
+ * 2: 8x32), tile order v / 3 (0: each XCD takes a contiguous range of
+ * tiles, 1: linear); particle-kernel workgroup size 256 or 512.  Results do
+ * not depend on either. */
+#define AMSC_P2G_VARIANTS 6
 int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block);
-/* Autotuner: time every tile shape, then every workgroup size, over
+/* Autotuner: time every P2G variant, then every workgroup size, over
  * steps_per_variant live steps each and keep the fastest.  The steps are
- * real steps of the run; *steps_used (may be NULL) receives their number. */
+ * real steps of the run; *steps_used (may be NULL) receives their number;
+ * variant_ms (may be NULL, AMSC_P2G_VARIANTS + 2 entries) the average P2G
+ * time of each variant, then the particle-kernel time of each size. */
 int amsc_sim_autotune(amsc_sim *s, int steps_per_variant, int *p2g_shape,
-                      int *particle_block, int *steps_used);
+                      int *particle_block, int *steps_used,
+                      double *variant_ms);
+/* Average duration (ms) of reps back-to-back P2G launches on the current
+ * state; they rewrite the nodal velocities the next step recomputes. */
+int amsc_sim_time_p2g(amsc_sim *s, int reps, double *avg_ms);
 
 /* ------------------------------------------------------------------ 4. */
 #define AMSC_PREC_FP64 0  /* every particle/nodal value and operation FP64 */
@@ -330,6 +339,18 @@ int amsc_ens_ms_counts(amsc_ens *e, int scen, int out[2]);
 /* Scenarios, precision, device bytes held, particle capacity. */
 int amsc_ens_info(amsc_ens *e, int *nscen, int *precision, long long *bytes,
                   int *capacity);
+/* Served form: a new batch holding the scenarios keep[0..nkeep) of e (their
+ * device state copied device-to-device, in that order) followed by nadd new
+ * scenarios given as for amsc_ens_create (their momentum must be vpx*Mp).
+ * On success e is destroyed and *out replaces it; on failure e is
+ * untouched.  Old and new batch are both allocated during the call. */
+int amsc_ens_repack(amsc_ens *e, int nkeep, const int *keep, int nadd,
+                    const amsc_params *params, const int *np, const double *x,
+                    const double *y, const double *Mp, const double *Ap,
+                    const double *vpx, const double *vpy,
+                    const double *mom_px, const double *mom_py,
+                    const double *hp, const double *Vp, const double *Z,
+                    long long capacity, amsc_ens **out);
 /* Device bytes amsc_ens_create would allocate (admission control); -1 on
  * bad arguments. */
 long long amsc_ens_bytes_estimate(int nscen, const amsc_params *params,

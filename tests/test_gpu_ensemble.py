@@ -132,7 +132,7 @@ def test_tile_shapes_and_blocks_give_identical_results():
     print("[ens] tilings", flush=True)
     d = small_slope_case(nrows=40, ncols=37, hx=0.25, hy=0.3)
     ref = None
-    for shape in (0, 1, 2):
+    for shape in range(6):
         for blk in (256, 512):
             s = Simulation.from_data(d)
             s.set_tiling(shape, blk)
@@ -154,7 +154,7 @@ def test_autotune_keeps_oracle_parity():
     d = small_slope_case()
     s = Simulation.from_data(d)
     shape, blk, used = s.autotune(2)
-    assert shape in (0, 1, 2) and blk in (256, 512) and used == 10
+    assert shape in range(6) and blk in (256, 512) and used == 16
     o = oracle.OracleSim(params_from_data(d), particles_from_data(d), d["Z"])
     for _ in range(used + 3):
         o.step()
@@ -293,3 +293,68 @@ def test_bytes_estimate_covers_allocation():
         held = e.info()["bytes"]
         assert held <= est <= 1.2 * held + (1 << 20), (prec, held, est)
         e.close()
+
+
+def test_repack_keeps_running_scenarios_bitwise():
+    """Scenarios kept through a device-side repack (with others removed and
+    new ones added) continue exactly as if run alone."""
+    from amsc_amd.simulation import Ensemble, Simulation
+    from amsc_amd import native
+    import ctypes as C
+    print("[ens] repack", flush=True)
+    cases = mixed_cases()
+    e = Ensemble(cases[:3], "fp64", capacity=8 * sum(len(p["x"]) for _, p, _ in cases))
+    e.step(7)
+    prm, part, Z = cases[3]
+    P = (native.AmscParams * 1)(native.AmscParams.from_dict(prm))
+    arrs = [np.ascontiguousarray(part[k], np.float64) for k in
+            ("x", "y", "Mp", "Ap", "vpx", "vpy", "mom_px", "mom_py", "hp", "Vp")]
+    h = C.c_void_p()
+    native.check(native.lib().amsc_ens_repack(
+        e._h, 2, np.array([2, 0], np.int32), 1, P, np.array([len(part["x"])], np.int32),
+        *arrs, np.ascontiguousarray(Z, np.float64), 0, C.byref(h)))
+    e._h = h
+    e.nscen, e.prms = 3, [cases[2][0], cases[0][0], cases[3][0]]
+    e.nns = [(p["nrows"] + 1) * (p["ncols"] + 1) for p in e.prms]
+    e.ncs = [p["nrows"] * p["ncols"] for p in e.prms]
+    e.step(11)
+    e.set_aux(True)
+    e.step(1)
+    for k, (c, steps) in enumerate(((cases[2], 19), (cases[0], 19), (cases[3], 12))):
+        s = Simulation(*c, capacity=8 * len(c[1]["x"]))
+        s.step(steps - 1)
+        s.set_aux(True)
+        s.step(1)
+        assert s.num_particles == e.num(k)
+        for f in FIELDS:
+            assert np.array_equal(bits(s.get(f)), bits(e.get(k, f))), (k, f)
+        assert s.state() == (int(e.status()["it"][k]), e.status()["t"][k],
+                             e.status()["dt"][k])
+        s.close()
+    e.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_served_requests_match_standalone_runs(graph):
+    """Poisson arrivals served by the batched FP64 form (joins and leaves
+    through repacks): every request ends in the state of its standalone run."""
+    from amsc_amd import serve
+    from amsc_amd.simulation import Simulation
+    print(f"[ens] served graph={graph}", flush=True)
+    datas = [scenarios.ensemble_scenario(40 + i, side_range=(24, 48), np_range=(300, 2000))
+             for i in range(6)]
+    arrivals = serve.poisson_arrivals(200.0, len(datas), seed=1)
+    reqs = serve.make_requests(datas, arrivals, max_steps=0)
+    for i, r in enumerate(reqs):
+        r.prm["max_steps"] = 15 + 7 * i
+    srv = serve.BatchedServer("fp64", chunk=4, graph=graph, collect=("x", "hp", "vpx"))
+    out = srv.run(reqs)
+    print(f"[ens] served: {out}", flush=True)
+    assert out["requests"] == len(reqs) and out["repacks"] >= 1
+    for r in reqs:
+        assert r.steps == r.prm["max_steps"]
+        s = Simulation(r.prm, r.particles, r.Z, capacity=4 * r.np)
+        s.run(50)
+        for f in ("x", "hp", "vpx"):
+            assert np.array_equal(bits(s.get(f)), bits(r.result[f])), (r.rid, f)
+        s.close()
