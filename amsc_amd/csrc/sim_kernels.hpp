@@ -108,6 +108,15 @@ struct Scen {
   int toff, ntiles, ntx;  // first P2G tile, tiles, tiles along x
 };
 
+// Arithmetic type of a storage type: FP64 computes in FP64; the mixed form
+// computes in FP32 (or in FP64 over FP32 storage when built with
+// AMSC_MIXED_FP64_MATH).
+template <typename R> struct ArithT { using type = R; };
+#ifdef AMSC_MIXED_FP64_MATH
+template <> struct ArithT<float> { using type = double; };
+#endif
+template <typename R> using Arith = typename ArithT<R>::type;
+
 template <typename R> __device__ __forceinline__ RecipT<R> recip_of(const Scen &S);
 template <>
 __device__ __forceinline__ RecipT<double> recip_of<double>(const Scen &S) {
@@ -358,7 +367,8 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   const double hx = sc.P.hx, hy = sc.P.hy, grav = sc.P.g, rho = sc.P.rho;
   const int coff = sc.coff, noff = sc.noff, toff = sc.toff, ntx = sc.ntx;
   const Recip rc = sc.rc;
-  const RecipT<R> rcr = recip_of<R>(sc);
+  using A = Arith<R>;
+  const RecipT<A> rcr = recip_of<A>(sc);
   const int lt = tile - toff;
   const int c0 = (lt % ntx) * TC, r0 = (lt / ntx) * TR;
   const int t = threadIdx.x;
@@ -419,7 +429,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
 
   double pm = 0, ppx = 0, ppy = 0;  // particle sums (owned cells)
   double Mv = 0, mvx = 0, mvy = 0;
-  R FPx = 0, FPy = 0, Frx = 0, Fry = 0, Fix = 0, Fiy = 0;
+  A FPx = 0, FPy = 0, Frx = 0, Fry = 0, Fix = 0, Fiy = 0;
 
   if (L > 0) {
     P2GIdx I1, I2, I3;
@@ -529,17 +539,17 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
           Mv += N * (double)s_n[0][i];
           mvx += N * s_w[2][i];
           mvy += N * s_w[3][i];
-          const R Nr = (R)N;
-          FPx += Nr * s_n[1][i];
-          FPy += Nr * s_n[2][i];
-          Frx += Nr * s_n[3][i];
-          Fry += Nr * s_n[4][i];
-          const R xr = (R)xi, er = (R)eta;
-          const R Nx = shape_dx_t<R>(xr, er, corner, rcr);
-          const R Ny = shape_dy_t<R>(xr, er, corner, rcr);
-          const R vp = s_n[8][i];
-          Fix += (Nx * s_n[5][i] + Ny * s_n[6][i]) * vp;
-          Fiy += (Nx * s_n[6][i] + Ny * s_n[7][i]) * vp;
+          const A Nr = (A)N;
+          FPx += Nr * (A)s_n[1][i];
+          FPy += Nr * (A)s_n[2][i];
+          Frx += Nr * (A)s_n[3][i];
+          Fry += Nr * (A)s_n[4][i];
+          const A xr = (A)xi, er = (A)eta;
+          const A Nx = shape_dx_t<A>(xr, er, corner, rcr);
+          const A Ny = shape_dy_t<A>(xr, er, corner, rcr);
+          const A vp = s_n[8][i];
+          Fix += (Nx * (A)s_n[5][i] + Ny * (A)s_n[6][i]) * vp;
+          Fiy += (Nx * (A)s_n[6][i] + Ny * (A)s_n[7][i]) * vp;
         }
       }
       __syncthreads();
@@ -552,8 +562,8 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     const int n = noff + r + c * (nrows + 1);
     const double dt = ctrl->dt;
     // force balance and nodal velocities, optim_par.cpp:467-497 (D1)
-    const R Fex = FPx + Frx, Fey = FPy + Fry;
-    const R Ftx = Fex - Fix, Fty = Fey - Fiy;
+    const A Fex = FPx + Frx, Fey = FPy + Fry;
+    const A Ftx = Fex - Fix, Fty = Fey - Fiy;
     double mx = dt * (double)Ftx + mvx, my = dt * (double)Fty + mvy;
     double ax = Mv > 1.e-2 ? (double)Ftx / Mv : 0.0;
     double ay = Mv > 1.e-2 ? (double)Fty / Mv : 0.0;
@@ -666,6 +676,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
                                               double dt, int ip, int &newkey,
                                               double &vmx, double &vmy,
                                               double &hmx) {
+  using A = Arith<R>;
   const KParams &P = sc.P;
   const PPtr &p = a.p;
   const int nrows = P.nrows;
@@ -675,7 +686,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   int nid[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) nid[k] = sc.noff + node_id(k, c, r, nrows);
-  R gx[4], gy[4];
+  A gx[4], gy[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     gx[k] = static_cast<const R *>(a.vvxL)[nid[k]];
@@ -684,16 +695,16 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   const double x = fld<double>(p, PF_X)[ip], y = fld<double>(p, PF_Y)[ip];
   const Local L0 = local_coords(x, y, c, r, hx, hy);
   // g2p(vvxL, vvyL), optim_par.cpp:542
-  const R xi0 = (R)L0.xi, eta0 = (R)L0.eta;
-  R vxL = 0, vyL = 0;
+  const A xi0 = (A)L0.xi, eta0 = (A)L0.eta;
+  A vxL = 0, vyL = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const R N = shape_t<R>(xi0, eta0, k);
+    const A N = shape_t<A>(xi0, eta0, k);
     vxL += N * gx[k];
     vyL += N * gy[k];
   }
   // PIC velocity, advection and clamp, optim_par.cpp:548-564
-  const R vpx = vxL, vpy = vyL;
+  const A vpx = vxL, vpy = vyL;
   double xn = x + dt * (double)vpx, yn = y + dt * (double)vpy;
   const double eps = 1e-10, Lx = P.ncols * P.hx, Ly = P.nrows * P.hy;
   if (xn < eps) xn = eps;
@@ -703,78 +714,78 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   // g2pd(vvxL, vvyL) at the new position, step-start cell (D18),
   // optim_par.cpp:578
   const Local L1 = local_coords(xn, yn, c, r, hx, hy);
-  const R xi1 = (R)L1.xi, eta1 = (R)L1.eta;
-  const RecipT<R> rcr = recip_of<R>(sc);
-  R vxdx = 0, vxdy = 0, vydx = 0, vydy = 0;
+  const A xi1 = (A)L1.xi, eta1 = (A)L1.eta;
+  const RecipT<A> rcr = recip_of<A>(sc);
+  A vxdx = 0, vxdy = 0, vydx = 0, vydy = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const R Nx = shape_dx_t<R>(xi1, eta1, k, rcr);
-    const R Ny = shape_dy_t<R>(xi1, eta1, k, rcr);
+    const A Nx = shape_dx_t<A>(xi1, eta1, k, rcr);
+    const A Ny = shape_dy_t<A>(xi1, eta1, k, rcr);
     vxdx += Nx * gx[k];
     vxdy += Ny * gx[k];
     vydx += Nx * gy[k];
     vydy += Ny * gy[k];
   }
   // height update with floor, area, optim_par.cpp:595-607 (FP64)
-  const R div = vxdx + vydy;
+  const A div = vxdx + vydy;
   double h = fld<double>(p, PF_HP)[ip] / (1 + dt * (double)div);
   h = h < 1e-2 ? 1e-2 : h;
-  const R Vp = fld<R>(p, PF_VP)[ip];
-  const R hR = (R)h;
-  const R Ap = h > 1.e-2 ? Vp / hR : R(0);
-  const R rho = (R)P.rho, g = (R)P.g;
+  const A Vp = fld<R>(p, PF_VP)[ip];
+  const A hR = (A)h;
+  const A Ap = h > 1.e-2 ? Vp / hR : A(0);
+  const A rho = (A)P.rho, g = (A)P.g;
   // Voellmy basal friction, optim_par.cpp:613-626
-  R Fbx = 0, Fby = 0;
+  A Fbx = 0, Fby = 0;
   {
-    const R nv = sqrt(vpx * vpx + vpy * vpy);
-    if (P.fric > 0 && nv > R(1e-10) && P.xi > 0) {
-      const R v2 = vpx * vpx + vpy * vpy;
-      const R fric = (R)P.fric, tan_fa = (R)P.tan_fa, xi = (R)P.xi;
+    const A nv = sqrt(vpx * vpx + vpy * vpy);
+    if (P.fric > 0 && nv > A(1e-10) && P.xi > 0) {
+      const A v2 = vpx * vpx + vpy * vpy;
+      const A fric = (A)P.fric, tan_fa = (A)P.tan_fa, xi = (A)P.xi;
       Fbx = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpx / nv;
       Fby = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpy / nv;
     }
   }
   // Bingham stress, stress_tensor_t::operator(), optim_par.cpp:68-120
-  R F11, F12, F22;
+  A F11, F12, F22;
   {
-    const R A = R(3) / R(2), C = R(65) / R(32);
-    const R mu = (R)P.mu, tau_Y = (R)P.tauy;
-    const R nrm = sqrt(vpx * vpx + vpy * vpy);
-    const R ALF = hR > R(1.e-3) ? (R(6) * mu * nrm) / ((hR + R(0.001)) * tau_Y) : R(0);
-    const R B = R(-114) / R(32) - ALF;
+    const A qa = A(3) / A(2), qc = A(65) / A(32);
+    const A mu = (A)P.mu, tau_Y = (A)P.tauy;
+    const A nrm = sqrt(vpx * vpx + vpy * vpy);
+    const A ALF = hR > A(1.e-3) ? (A(6) * mu * nrm) / ((hR + A(0.001)) * tau_Y) : A(0);
+    const A B = A(-114) / A(32) - ALF;
     // Z1 = (-B + sq)/(2A) is selected only if it lies in [0, 1]; when
     // -B + sq > 3.0000001 it cannot (2A = 3), so Z2 is taken without
     // evaluating Z1.  Same values as computing both.
-    const R sq = sqrt(B * B - R(4) * A * C);
-    const R num1 = -B + sq;
-    R ZZ;
-    if (num1 > R(3.0000001)) {
-      ZZ = (-B - sq) / (R(2) * A);
+    const A sq = sqrt(B * B - A(4) * qa * qc);
+    const A num1 = -B + sq;
+    A ZZ;
+    if (num1 > A(3.0000001)) {
+      ZZ = (-B - sq) / (A(2) * qa);
     } else {
-      const R Z1 = num1 / (R(2) * A);
-      ZZ = fabs(Z1 - R(.5)) <= R(.5) ? Z1 : (-B - sq) / (R(2) * A);
+      const A Z1 = num1 / (A(2) * qa);
+      ZZ = fabs(Z1 - A(.5)) <= A(.5) ? Z1 : (-B - sq) / (A(2) * qa);
     }
-    const R D_xx = vxdx;
-    const R D_xy = R(0.5) * (vxdy + vydx);
-    const R D_yy = vydy;
-    const R D_zx = hR > R(1.e-3) ? R(0.5) * (R(3) / (R(2) + ZZ)) * (vpx / (hR + R(0.001))) : R(0);
-    const R D_zy = hR > R(1.e-3) ? R(0.5) * (R(3) / (R(2) + ZZ)) * (vpy / (hR + R(0.001))) : R(0);
-    const R D_zz = -(vxdx + vydy);
-    const R invII = R(0.5) * (D_xx * D_xx + D_yy * D_yy + D_zz * D_zz) +
+    const A D_xx = vxdx;
+    const A D_xy = A(0.5) * (vxdy + vydx);
+    const A D_yy = vydy;
+    const A D_zx = hR > A(1.e-3) ? A(0.5) * (A(3) / (A(2) + ZZ)) * (vpx / (hR + A(0.001))) : A(0);
+    const A D_zy = hR > A(1.e-3) ? A(0.5) * (A(3) / (A(2) + ZZ)) * (vpy / (hR + A(0.001))) : A(0);
+    const A D_zz = -(vxdx + vydy);
+    const A invII = A(0.5) * (D_xx * D_xx + D_yy * D_yy + D_zz * D_zz) +
                     D_zx * D_zx + D_zy * D_zy + D_xy * D_xy;
-    const R sig_xx = invII != R(0) ? (tau_Y / sqrt(invII) + R(2) * mu) * D_xx : R(0);
-    const R sig_xy = invII != R(0) ? (tau_Y / sqrt(invII) + R(2) * mu) * D_xy : R(0);
-    const R sig_yy = invII != R(0) ? (tau_Y / sqrt(invII) + R(2) * mu) * D_yy : R(0);
+    const A sig_xx = invII != A(0) ? (tau_Y / sqrt(invII) + A(2) * mu) * D_xx : A(0);
+    const A sig_xy = invII != A(0) ? (tau_Y / sqrt(invII) + A(2) * mu) * D_xy : A(0);
+    const A sig_yy = invII != A(0) ? (tau_Y / sqrt(invII) + A(2) * mu) * D_yy : A(0);
     // hydrostatic part with the well-balance correction, FP64
     const double H = fld<double>(p, PF_H)[ip];
     const double h_corr = h > 1e-10 ? h - (H * H) / h : 0.0;
-    if (sizeof(R) == sizeof(double)) {
+    if (sizeof(A) == sizeof(double)) {
       F11 = P.bing * sig_xx - .5 * P.rho * P.g * h_corr;
       F12 = P.bing * sig_xy;
       F22 = P.bing * sig_yy - .5 * P.rho * P.g * h_corr;
     } else {
-      const R pres = (R)(.5 * P.rho * P.g * h_corr);
-      const R bing = (R)P.bing;
+      const A pres = (A)(.5 * P.rho * P.g * h_corr);
+      const A bing = (A)P.bing;
       F11 = bing * sig_xx - pres;
       F12 = bing * sig_xy;
       F22 = bing * sig_yy - pres;
@@ -810,11 +821,11 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) zp += shape(L1, k) * a.Z[nid[k]];
-    fld<R>(p, PF_APX)[ip] = (R)ax;
-    fld<R>(p, PF_APY)[ip] = (R)ay;
-    fld<R>(p, PF_ZP)[ip] = (R)zp;
-    fld<R>(p, PF_HPZ)[ip] = (R)(h + zp);
-    const R Mp = fld<R>(p, PF_MP)[ip];
+    fld<R>(p, PF_APX)[ip] = (A)ax;
+    fld<R>(p, PF_APY)[ip] = (A)ay;
+    fld<R>(p, PF_ZP)[ip] = (A)zp;
+    fld<R>(p, PF_HPZ)[ip] = (A)(h + zp);
+    const A Mp = fld<R>(p, PF_MP)[ip];
     fld<R>(p, PF_MOMX)[ip] = vpx * Mp;
     fld<R>(p, PF_MOMY)[ip] = vpy * Mp;
   }

@@ -152,11 +152,17 @@ def run_headline(args, d, precision):
     else:
         sim = Ensemble([(prm, particles_from_data(d), d["Z"])], "mixed", capacity=3 * np0)
     tune = {}
-    variant, block, used = sim.autotune(2, tune)
+    if args.tiling:
+        variant, block = (int(v) for v in args.tiling.split(","))
+        sim.set_tiling(variant, block)
+        used = 0
+    else:
+        variant, block, used = sim.autotune(2, tune)
     rest = max(args.warmup - used, 0)
     sim.step(rest)
-    log(f"[bench] {precision}: autotune -> P2G variant {variant}, particle block "
-        f"{block} ({used} steps), warmup {rest} more steps")
+    log(f"[bench] {precision}: P2G variant {variant}, particle block {block} "
+        f"({'autotuned over ' + str(used) + ' steps' if used else 'fixed'}), "
+        f"warmup {rest} more steps")
     ps0 = sim.particle_steps
     t0 = time.perf_counter()
     done = sim.step(args.steps)
@@ -252,13 +258,15 @@ def served_bench(args):
                  for i in range(args.served_requests)]
         arr = serve.poisson_arrivals(rate, len(datas), seed=int(rate))
         res = {}
+        log(f"[bench] served {rate}/s: {len(datas)} requests, "
+            f"{sum(len(d['x']) for d in datas)} particles")
         for form in ("mixed", "fp64"):
             reqs = serve.make_requests(datas, arr, max_steps=args.served_max_steps)
             if form == "mixed":
                 srv = serve.BatchedServer("mixed", budget_bytes=int(0.8 * HBM_BYTES))
             else:
                 srv = serve.SequentialServer()
-            r = srv.run(reqs)
+            r = srv.run(reqs, log=log)
             r["steps_mean"] = float(np.mean([q.steps for q in reqs]))
             res[form] = r
             log(f"[bench] served {rate}/s {form}: {r['scen_per_s']:.2f} scen/s, "
@@ -282,8 +290,11 @@ def main():
     ap.add_argument("--ens-scenarios", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--served", action="store_true")
-    ap.add_argument("--served-requests", type=int, default=40)
-    ap.add_argument("--served-max-steps", type=int, default=0)
+    ap.add_argument("--served-requests", type=int, default=24)
+    ap.add_argument("--served-max-steps", type=int, default=400,
+                    help="step cap per request (0: run to the seeded end time)")
+    ap.add_argument("--tiling", default="", metavar="V,B",
+                    help="fixed P2G variant and particle block instead of autotuning")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/"
