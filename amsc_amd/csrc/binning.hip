@@ -5,125 +5,21 @@
 // Pipeline (all integer, so results are exact and run-to-run identical):
 //   1. key = cell of the particle, slot = atomicAdd(count[key]) aggregated
 //      per wavefront (fused into the particle-update kernel of a time step);
-//   2. cstart = exclusive scan of count (count reset to 0 on the way);
+//   2. cstart[1..nc] = inclusive scan of count (one single-pass decoupled
+//      look-back scan, rocPRIM), cstart[0] = 0 from allocation;
 //   3. cptcls[cstart[key] + slot] = id;
 //   4. every cell's slice of cptcls sorted ascending: atomics fix which
 //      slots a cell receives but not their order, so this step restores the
-//      stable order of a CPU counting sort (ascending id within a cell).
+//      stable order of a CPU counting sort (ascending id within a cell);
+//      the cell's count is reset to 0 for the next binning.
 #include "binning.hpp"
 #include "amsc_mpm.h"
+
+#include <rocprim/device/device_scan.hpp>
 
 namespace amsc {
 
 namespace {
-
-constexpr int kScanBlock = 256;
-constexpr int kScanItems = 16;
-constexpr int kScanChunk = kScanBlock * kScanItems;  // 4096
-
-// Exclusive scan of one value per thread over a 256-thread block.  Returns
-// the exclusive prefix; *total receives the block total.
-__device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
-  __shared__ int s_wave[kScanBlock / 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int t = __shfl_up(inc, d);
-    if (lane >= d) inc += t;
-  }
-  if (lane == 63) s_wave[wave] = inc;
-  __syncthreads();
-  int wbase = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < kScanBlock / 64; ++w) {
-    if (w < wave) wbase += s_wave[w];
-    tot += s_wave[w];
-  }
-  __syncthreads();
-  *total = tot;
-  return wbase + inc - v;
-}
-
-__global__ void __launch_bounds__(kScanBlock)
-    scan_partials_kernel(const int *__restrict__ in, int n, int *bsum,
-                         const int *__restrict__ active) {
-  if (active && !*active) return;
-  const long long base = (long long)blockIdx.x * kScanChunk;
-  int s = 0;
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const long long j = base + (long long)i * kScanBlock + threadIdx.x;
-    if (j < n) s += in[j];
-  }
-  int total;
-  (void)block_exclusive_scan(s, &total);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-// Single block: exclusive scan of bsum[0..nb) in place, total in bsum[nb].
-__global__ void __launch_bounds__(kScanBlock)
-    scan_bsums_kernel(int *bsum, int nb, const int *__restrict__ active) {
-  if (active && !*active) return;
-  int carry = 0;
-  for (int base = 0; base < nb; base += kScanBlock) {
-    const int j = base + threadIdx.x;
-    const int v = j < nb ? bsum[j] : 0;
-    int total;
-    const int ex = block_exclusive_scan(v, &total);
-    if (j < nb) bsum[j] = carry + ex;
-    carry += total;
-  }
-  if (threadIdx.x == 0) bsum[nb] = carry;
-}
-
-__global__ void __launch_bounds__(kScanBlock)
-    scan_down_kernel(int *in, int *out, int n, const int *bsum,
-                     int zero_in, const int *__restrict__ active) {
-  if (active && !*active) return;
-  __shared__ int s_v[kScanChunk];
-  const long long base = (long long)blockIdx.x * kScanChunk;
-  // coalesced load, then each thread owns kScanItems consecutive values
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const int l = i * kScanBlock + threadIdx.x;
-    const long long j = base + l;
-    s_v[l] = j < n ? in[j] : 0;
-    if (zero_in && j < n) in[j] = 0;
-  }
-  __syncthreads();
-  int vals[kScanItems];
-  int s = 0;
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    vals[i] = s_v[threadIdx.x * kScanItems + i];
-    s += vals[i];
-  }
-  int total;
-  int run = block_exclusive_scan(s, &total) + bsum[blockIdx.x];
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    s_v[threadIdx.x * kScanItems + i] = run;
-    run += vals[i];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const int l = i * kScanBlock + threadIdx.x;
-    const long long j = base + l;
-    if (j < n) out[j] = s_v[l];
-  }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-    const int nb = gridDim.x;
-    out[n] = bsum[nb];
-  }
-}
-
-__global__ void set_zero_kernel(int *p, const int *__restrict__ active) {
-  if (active && !*active) return;
-  *p = 0;
-}
 
 __global__ void key_count_kernel(int np, const double *__restrict__ x,
                                  const double *__restrict__ y, int nrows,
@@ -155,10 +51,12 @@ __global__ void place_kernel(int np, const int *__restrict__ np_dev,
 constexpr int kSortLocal = 32;
 
 __global__ void cell_sort_kernel(int nc, const int *__restrict__ cstart,
-                                 int *cptcls, const int *__restrict__ active) {
+                                 int *cptcls, int *count,
+                                 const int *__restrict__ active) {
   if (active && !*active) return;
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= nc) return;
+  count[cell] = 0;
   const int b = cstart[cell], e = cstart[cell + 1], n = e - b;
   if (n < 2) return;
   if (n <= kSortLocal) {
@@ -207,35 +105,34 @@ __global__ void colouring_kernel(int nrows, int ncols, int *ccidx, int o1,
 }  // namespace
 
 int scan_reserve(ScanWork &w, int n) {
-  const int nb = (int)((n + (long long)kScanChunk - 1) / kScanChunk);
-  if (w.nb_cap >= nb + 1) return AMSC_OK;
-  if (w.bsum) (void)hipFree(w.bsum);
-  w.bsum = nullptr;
-  w.nb_cap = 0;
-  if (hipMalloc(&w.bsum, sizeof(int) * (nb + 1)) != hipSuccess)
+  size_t bytes = 0;
+  const hipError_t e = rocprim::inclusive_scan(
+      nullptr, bytes, static_cast<const int *>(nullptr),
+      static_cast<int *>(nullptr), (size_t)(n > 0 ? n : 1),
+      rocprim::plus<int>());
+  if (e != hipSuccess)
+    return fail(AMSC_ERR_HIP, std::string("scan sizing: ") + hipGetErrorString(e));
+  if (w.bytes >= bytes && w.tmp) return AMSC_OK;
+  if (w.tmp) (void)hipFree(w.tmp);
+  w.tmp = nullptr;
+  w.bytes = 0;
+  if (hipMalloc(&w.tmp, bytes) != hipSuccess)
     return fail(AMSC_ERR_NOMEM, "scan workspace allocation failed");
-  w.nb_cap = nb + 1;
+  w.bytes = bytes;
   return AMSC_OK;
 }
 
-int exclusive_scan(const int *in, int *out, int n, bool zero_in, ScanWork &w,
-                   hipStream_t st, int *total_host, const int *active) {
-  if (n <= 0) {
-    set_zero_kernel<<<1, 1, 0, st>>>(out, active);
-    AMSC_LAUNCH_CHECK();
-    if (total_host) {
-      AMSC_HIP(hipStreamSynchronize(st));
-      *total_host = 0;
-    }
-    return AMSC_OK;
+int exclusive_scan(const int *in, int *out, int n, ScanWork &w,
+                   hipStream_t st, int *total_host) {
+  if (n > 0) {
+    if (int rc = scan_reserve(w, n)) return rc;
+    size_t bytes = w.bytes;
+    const hipError_t e = rocprim::inclusive_scan(w.tmp, bytes, in, out + 1,
+                                                 (size_t)n, rocprim::plus<int>(),
+                                                 st);
+    if (e != hipSuccess)
+      return fail(AMSC_ERR_HIP, std::string("scan: ") + hipGetErrorString(e));
   }
-  const int nb = (int)((n + (long long)kScanChunk - 1) / kScanChunk);
-  if (int rc = scan_reserve(w, n)) return rc;
-  scan_partials_kernel<<<nb, kScanBlock, 0, st>>>(in, n, w.bsum, active);
-  scan_bsums_kernel<<<1, kScanBlock, 0, st>>>(w.bsum, nb, active);
-  scan_down_kernel<<<nb, kScanBlock, 0, st>>>(const_cast<int *>(in), out, n,
-                                              w.bsum, zero_in ? 1 : 0, active);
-  AMSC_LAUNCH_CHECK();
   if (total_host) {
     AMSC_HIP(hipMemcpyAsync(total_host, out + n, sizeof(int),
                             hipMemcpyDeviceToHost, st));
@@ -259,7 +156,7 @@ int binning_alloc(Binning &b, int nrows, int ncols) {
 void binning_free(Binning &b) {
   if (b.count) (void)hipFree(b.count);
   if (b.cstart) (void)hipFree(b.cstart);
-  if (b.scan.bsum) (void)hipFree(b.scan.bsum);
+  if (b.scan.tmp) (void)hipFree(b.scan.tmp);
   b = Binning();
 }
 
@@ -277,16 +174,15 @@ int launch_finish_binning(Binning &b, int np, const int *key,
                           const int *slot, int *cptcls, const int *active,
                           hipStream_t st, const int *np_dev) {
   const int nc = b.nrows * b.ncols;
-  if (int rc = exclusive_scan(b.count, b.cstart, nc, true, b.scan, st,
-                              nullptr, active))
+  if (active) return fail(AMSC_ERR_INVALID, "binning: unsupported gating");
+  if (int rc = exclusive_scan(b.count, b.cstart, nc, b.scan, st, nullptr))
     return rc;
-  if (np > 0) {
+  if (np > 0)
     place_kernel<<<grid_for(np, 256), 256, 0, st>>>(np, np_dev, key, slot,
                                                     b.cstart, cptcls, active);
-    cell_sort_kernel<<<grid_for(nc, 256), 256, 0, st>>>(nc, b.cstart, cptcls,
-                                                        active);
-    AMSC_LAUNCH_CHECK();
-  }
+  cell_sort_kernel<<<grid_for(nc, 256), 256, 0, st>>>(nc, b.cstart, cptcls,
+                                                      b.count, active);
+  AMSC_LAUNCH_CHECK();
   return AMSC_OK;
 }
 

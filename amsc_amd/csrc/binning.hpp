@@ -5,19 +5,18 @@
 
 namespace amsc {
 
-// Reusable device workspace for the 3-phase exclusive scan.
+// Device workspace of the single-pass scan.
 struct ScanWork {
-  int *bsum = nullptr;  // per-block sums
-  int nb_cap = 0;
+  void *tmp = nullptr;
+  size_t bytes = 0;
 };
 
-// out[0..n] = exclusive prefix of in[0..n), out[n] = total.  If zero_in,
-// in[] is reset to 0 after it is read.  If total_host != nullptr the total
-// is copied back (synchronising the stream).
-// The kernels do nothing when *active == 0 (active may be null).
-int exclusive_scan(const int *in, int *out, int n, bool zero_in,
-                   ScanWork &w, hipStream_t st, int *total_host,
-                   const int *active = nullptr);
+// out[1..n] = inclusive prefix of in[0..n); out[0] is not written and must
+// already hold 0, so out[0..n] is the exclusive prefix with the total in
+// out[n].  If total_host != nullptr the total is copied back (synchronising
+// the stream).
+int exclusive_scan(const int *in, int *out, int n, ScanWork &w,
+                   hipStream_t st, int *total_host);
 
 // Size the workspace for scans of up to n values (a scan of a size already
 // reserved allocates nothing, so it may be captured into a graph).
@@ -27,7 +26,7 @@ int scan_reserve(ScanWork &w, int n);
 struct Binning {
   int nrows = 0, ncols = 0;
   int *count = nullptr;   // [nc] cell counts (kept zero between uses)
-  int *cstart = nullptr;  // [nc+1]
+  int *cstart = nullptr;  // [nc+1], cstart[0] = 0 from allocation
   ScanWork scan;
 };
 
@@ -39,8 +38,9 @@ int launch_key_count(int np, const double *x, const double *y, int nrows,
                      int ncols, double hx, double hy, int *key, int *slot,
                      int *count, const int *active, hipStream_t st);
 // cstart from counts, then cptcls from (key, slot), then ascending particle
-// ids within every cell.  With np_dev non-null the particle count is read
-// from device memory (np then only sizes the launch and must bound it).
+// ids within every cell, counts reset to 0.  With np_dev non-null the
+// particle count is read from device memory (np then only sizes the launch
+// and must bound it).  `active` must be null.
 int launch_finish_binning(Binning &b, int np, const int *key,
                           const int *slot, int *cptcls, const int *active,
                           hipStream_t st, const int *np_dev = nullptr);

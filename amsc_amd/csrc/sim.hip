@@ -13,8 +13,9 @@
 //                 staged through LDS and every node thread folds its cells
 //                 in the serial p2g order, then applies the nodal update and
 //                 boundary conditions in registers.
-//   conservation  one workgroup per scenario: err_mass, err_mom from the
-//                 tile partials.
+//   conservation  (on request, amsc_ens_conservation) one workgroup per
+//                 scenario: err_mass, err_mom of the last step from the P2G
+//                 tile partials, which stay on the device until then.
 //   particle      one thread per particle: G2P, advection, G2PD, height,
 //                 Voellmy, Bingham, friction force, new cell key with a
 //                 wave-aggregated count, CFL maxima.
@@ -33,7 +34,9 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace amsc {
@@ -106,10 +109,13 @@ struct Sim {
   std::vector<char> ended;     // run known to have ended, per scenario
   int tile_cfg = 0, tile_order = 0, part_bs = 256;
   long long particle_steps = 0;
-  size_t bytes = 0;  // device memory held
+  size_t bytes = 0;  // device memory requested (cached blocks may be larger)
+  std::vector<std::pair<void *, size_t>> allocs;  // every dalloc block
   bool finished = false, aux_on = false, diag = true, mom_stored = false;
   bool aux_valid = true, nodal_valid = false, timing = false;
   bool graph_on = false, broken = false;
+  bool cons_stale = false;  // a P2G ran since the conservation reduction
+  int cons_cfg = 0;         // tile shape of the partials' layout
   std::map<long long, GraphEntry> graphs;
   std::vector<TimedLaunch> launches;
   std::vector<hipEvent_t> event_pool;
@@ -181,13 +187,72 @@ Tables tables(const Sim &s) {
 size_t pf_bytes(const Sim &s, int k) { return pf_wide(k) ? 8 : s.rsz; }
 size_t nf_bytes(const Sim &s, int k) { return nf_narrow(k) ? s.rsz : 8; }
 
-template <typename T> int dalloc(Sim &s, T **p, size_t n) {
-  const size_t b = sizeof(T) * (n > 0 ? n : 1);
-  if (hipMalloc(p, b) != hipSuccess) {
-    *p = nullptr;
-    return fail(AMSC_ERR_NOMEM, "device allocation of " + std::to_string(b) +
-                                    " bytes failed");
+// Device memory of destroyed batches is kept in a cache and handed to the
+// next batch of the process: a served batch is rebuilt at every join and
+// leave, and hipMalloc/hipFree of its arrays would otherwise dominate the
+// rebuild.  A cached block serves a request of at least half its size; the
+// cache holds at most kCacheMax bytes (amsc_release_cache empties it).
+constexpr size_t kCacheMax = 32ull << 30;
+
+struct DevCache {
+  std::mutex m;
+  std::multimap<size_t, void *> blocks;
+  size_t held = 0;
+};
+
+DevCache &dev_cache() {
+  static DevCache c;
+  return c;
+}
+
+bool cache_take(size_t b, void **p, size_t *got) {
+  DevCache &c = dev_cache();
+  std::lock_guard<std::mutex> g(c.m);
+  auto it = c.blocks.lower_bound(b);
+  if (it == c.blocks.end() || it->first > 2 * b) return false;
+  *p = it->second;
+  *got = it->first;
+  c.held -= it->first;
+  c.blocks.erase(it);
+  return true;
+}
+
+void cache_give(void *p, size_t b) {
+  DevCache &c = dev_cache();
+  std::lock_guard<std::mutex> g(c.m);
+  if (c.held + b > kCacheMax) {
+    (void)hipFree(p);
+    return;
   }
+  c.blocks.emplace(b, p);
+  c.held += b;
+}
+
+void cache_release() {
+  DevCache &c = dev_cache();
+  std::lock_guard<std::mutex> g(c.m);
+  for (auto &kv : c.blocks) (void)hipFree(kv.second);
+  c.blocks.clear();
+  c.held = 0;
+}
+
+template <typename T> int dalloc(Sim &s, T **p, size_t n) {
+  const size_t b = (sizeof(T) * (n > 0 ? n : 1) + 255) & ~size_t(255);
+  void *q = nullptr;
+  size_t got = b;
+  if (!cache_take(b, &q, &got)) {
+    if (hipMalloc(&q, b) != hipSuccess) {
+      // give the cache back to the device and retry once
+      cache_release();
+      if (hipMalloc(&q, b) != hipSuccess) {
+        *p = nullptr;
+        return fail(AMSC_ERR_NOMEM, "device allocation of " +
+                                        std::to_string(b) + " bytes failed");
+      }
+    }
+  }
+  *p = static_cast<T *>(q);
+  s.allocs.emplace_back(q, got);
   s.bytes += b;
   return AMSC_OK;
 }
@@ -202,40 +267,15 @@ int alloc_ppt(Sim &s, PPtr &p) {
   return dalloc(s, &p.level, s.cap);
 }
 
-void free_ppt(PPtr &p) {
-  for (int k = 0; k < PF_COUNT; ++k)
-    if (p.f[k]) (void)hipFree(p.f[k]);
-  if (p.label) (void)hipFree(p.label);
-  if (p.level) (void)hipFree(p.level);
-  p = PPtr();
-}
-
 void sim_free(Sim *s) {
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
   for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.second.exec);
-  free_ppt(s->buf[0]);
-  free_ppt(s->buf[1]);
-  int *ip[] = {s->key[0], s->key[1], s->slot,     s->cptcls,  s->act,
-               s->partner, s->wts,   s->offs,     s->ms_cnt,  s->poff,
-               s->poff_next, s->wscen, s->cell_off, s->err};
-  for (int *q : ip)
-    if (q) (void)hipFree(q);
+  for (auto &a : s->allocs) cache_give(a.first, a.second);
   binning_free(s->bin);
-  if (s->ms_scan.bsum) (void)hipFree(s->ms_scan.bsum);
-  for (int k = 0; k < NF_COUNT; ++k)
-    if (s->nod.f[k]) (void)hipFree(s->nod.f[k]);
-  for (int c = 0; c < kTileCfgs; ++c) {
-    if (s->scen[c]) (void)hipFree(s->scen[c]);
-    if (s->tscen[c]) (void)hipFree(s->tscen[c]);
-  }
-  if (s->Z) (void)hipFree(s->Z);
-  if (s->ctrl) (void)hipFree(s->ctrl);
+  if (s->ms_scan.tmp) (void)hipFree(s->ms_scan.tmp);
   if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
   if (s->host_ints) (void)hipHostFree(s->host_ints);
-  if (s->partials) (void)hipFree(s->partials);
-  for (int b = 0; b < 2; ++b)
-    if (s->p2g_out[b]) (void)hipFree(s->p2g_out[b]);
   for (auto e : s->event_pool) (void)hipEventDestroy(e);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
@@ -376,8 +416,7 @@ int merge_split(Sim &s, bool capture) {
                           (const int *)s.poff, s.S, n, (const int *)s.act,
                           s.wts))
         return rc;
-    if (int rc = exclusive_scan(s.wts, s.offs, n, false, s.ms_scan, s.st,
-                                nullptr))
+    if (int rc = exclusive_scan(s.wts, s.offs, n, s.ms_scan, s.st, nullptr))
       return rc;
   }
   if (int rc = launch(s, "ms_offsets", ms_offsets_kernel, dim3(1), b256,
@@ -510,11 +549,7 @@ int step_body(Sim &s, bool ms_step, int aux, bool capture, bool *skipped) {
   const Tables T = tables(s);
   PPtr &p = s.buf[s.cur];
   if (int rc = launch_p2g<R>(s, p2g_args(s, aux, s.diag))) return rc;
-  if (s.diag)
-    if (int rc = launch(s, "conservation", conservation_kernel, dim3(s.S),
-                        dim3(256), (const Scen *)s.scen[s.tile_cfg],
-                        (const double *)s.partials, s.ctrl))
-      return rc;
+
   const int n = capture ? s.cap : s.np;
   if (n > 0) {
     PartArgs a;
@@ -625,6 +660,8 @@ int launch_step(Sim &s) {
   s.aux_valid = aux == AUX_FULL;
   s.nodal_valid = aux == AUX_FULL;
   s.mom_stored = false;
+  s.cons_stale = s.diag;
+  s.cons_cfg = s.tile_cfg;
   bool all = true;
   for (int i = 0; i < s.S; ++i) {
     if (!s.ended[i]) {
@@ -686,10 +723,22 @@ int upload_narrow(Sim &s, void *dst, const double *src, size_t n,
                 (const double *)stage, static_cast<float *>(dst), (long long)n);
 }
 
+// err_mass, err_mom of the last step from the P2G tile partials, which stay
+// on the device until they are asked for.
+int reduce_conservation(Sim &s) {
+  if (!s.cons_stale) return AMSC_OK;
+  if (int rc = launch(s, "conservation", conservation_kernel, dim3(s.S),
+                      dim3(256), (const Scen *)s.scen[s.cons_cfg],
+                      (const double *)s.partials, s.ctrl))
+    return rc;
+  s.cons_stale = false;
+  return AMSC_OK;
+}
+
 // Layout, scenario tables and device allocations of a batch; zero nodal
 // arrays.  Particle data, Z and the control blocks are filled by the caller.
 int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
-          int precision, long long capacity) {
+          int precision, long long capacity, hipStream_t stream = nullptr) {
   if (S <= 0 || S > kMaxScen || !prm || !np_arr)
     return fail(AMSC_ERR_INVALID, "bad ensemble arguments");
   if (precision != AMSC_PREC_FP64 && precision != AMSC_PREC_MIXED)
@@ -758,7 +807,8 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   s->it_host.assign(S, 0);
   s->ended.assign(S, 0);
 
-  AMSC_HIP(hipStreamCreate(&s->st));
+  if (stream) s->st = stream;
+  else AMSC_HIP(hipStreamCreate(&s->st));
   for (int b = 0; b < 2; ++b)
     if (int rc = alloc_ppt(*s, s->buf[b])) return rc;
   int **ints[] = {&s->key[0], &s->key[1], &s->slot, &s->cptcls, &s->act,
@@ -814,6 +864,7 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
                   (long long)s->nn, 0.0f);
     if (rc) return rc;
   }
+  if (int rc = launch_fill_int(*s, s->offs, 1, 0)) return rc;  // offs[0]
   return launch_fill_int(*s, s->err, 1, 0);
 }
 
@@ -839,15 +890,12 @@ int upload_and_init(Sim *s, size_t p0, size_t n, const double *x,
                     const double *hp, const double *Vp) {
   hipStream_t st = s->st;
   PPtr &b = s->buf[0];
+  // staging (released with the batch, so a later rebuild reuses it)
   double *stage = nullptr, *mtmp[2] = {nullptr, nullptr};
   if (n > 0) {
-    if (hipMalloc(&stage, sizeof(double) * n) != hipSuccess ||
-        hipMalloc(&mtmp[0], sizeof(double) * n) != hipSuccess ||
-        hipMalloc(&mtmp[1], sizeof(double) * n) != hipSuccess) {
-      if (stage) (void)hipFree(stage);
-      if (mtmp[0]) (void)hipFree(mtmp[0]);
-      return fail(AMSC_ERR_NOMEM, "staging allocation failed");
-    }
+    if (int rc = dalloc(*s, &stage, n)) return rc;
+    if (int rc = dalloc(*s, &mtmp[0], n)) return rc;
+    if (int rc = dalloc(*s, &mtmp[1], n)) return rc;
   }
   int rc = AMSC_OK;
   if (n > 0) {
@@ -897,10 +945,6 @@ int upload_and_init(Sim *s, size_t p0, size_t n, const double *x,
   }
   if (!rc && hipStreamSynchronize(st) != hipSuccess)
     rc = fail(AMSC_ERR_HIP, "initialisation failed");
-  (void)hipStreamSynchronize(st);
-  if (stage) (void)hipFree(stage);
-  if (mtmp[0]) (void)hipFree(mtmp[0]);
-  if (mtmp[1]) (void)hipFree(mtmp[1]);
   return rc;
 }
 
@@ -956,7 +1000,9 @@ int repack(Sim *o, Sim *s, int nkeep, const int *keep, int nadd,
   for (int i = 0; i < nkeep; ++i)
     if (keep[i] < 0 || keep[i] >= o->S)
       return fail(AMSC_ERR_INVALID, "bad scenario index");
-  // host mirrors must be exact before copying
+  // host mirrors must be exact before copying; the kept control blocks
+  // carry their last conservation errors
+  if (int rc = reduce_conservation(*o)) return rc;
   AMSC_HIP(hipStreamSynchronize(o->st));
   if (int rc = read_poff(*o)) return rc;
   std::vector<amsc_params> prm;
@@ -978,45 +1024,60 @@ int repack(Sim *o, Sim *s, int nkeep, const int *keep, int nadd,
   if (nnew > 0 && momentum_stored(nnew, vpx, vpy, Mp, mom_px, mom_py))
     return fail(AMSC_ERR_INVALID,
                 "scenarios added to a running batch start from momentum vpx*Mp");
+  // the new batch takes over the old one's stream (ordering the copies
+  // after the old batch's work)
   if (int rc = setup(s, nkeep + nadd, prm.data(), nps.data(),
-                     o->mixed ? AMSC_PREC_MIXED : AMSC_PREC_FP64, capacity))
+                     o->mixed ? AMSC_PREC_MIXED : AMSC_PREC_FP64, capacity,
+                     o->st))
     return rc;
   hipStream_t st = s->st;
-  // finish the old batch's work before reading its buffers on another stream
-  AMSC_HIP(hipStreamSynchronize(o->st));
-  const PPtr &src = o->buf[o->cur];
-  PPtr &dst = s->buf[0];
-  for (int i = 0; i < nkeep; ++i) {
-    const int k = keep[i];
-    const size_t sp = o->poff_h[k], dp = s->poff_h[i], n = nps[i];
-    if (n > 0) {
-      for (int f = 0; f < PF_COUNT; ++f) {
-        const size_t e = pf_bytes(*s, f);
-        AMSC_HIP(hipMemcpyAsync(static_cast<char *>(dst.f[f]) + e * dp,
-                                static_cast<const char *>(src.f[f]) + e * sp,
-                                e * n, hipMemcpyDeviceToDevice, st));
-      }
-      AMSC_HIP(hipMemcpyAsync(dst.label + dp, src.label + sp, sizeof(int) * n,
-                              hipMemcpyDeviceToDevice, st));
-      AMSC_HIP(hipMemcpyAsync(dst.level + dp, src.level + sp, sizeof(int) * n,
-                              hipMemcpyDeviceToDevice, st));
+  if (nkeep > 0) {
+    // copy maps: new and old offsets of the kept scenarios, then keep[]
+    std::vector<int> tab(5 * nkeep + 2);
+    int *pd = tab.data(), *ps = pd + nkeep + 1, *nd = ps + nkeep,
+        *ns = nd + nkeep + 1, *kp = ns + nkeep;
+    for (int i = 0; i <= nkeep; ++i) {
+      pd[i] = s->poff_h[i];
+      nd[i] = s->noff_h[i];
     }
-    const size_t sn = o->noff_h[k], dn = s->noff_h[i];
-    const size_t nn = o->noff_h[k + 1] - o->noff_h[k];
-    for (int f = 0; f < NF_COUNT; ++f) {
-      const size_t e = nf_bytes(*s, f);
-      AMSC_HIP(hipMemcpyAsync(static_cast<char *>(s->nod.f[f]) + e * dn,
-                              static_cast<const char *>(o->nod.f[f]) + e * sn,
-                              e * nn, hipMemcpyDeviceToDevice, st));
+    for (int i = 0; i < nkeep; ++i) {
+      ps[i] = o->poff_h[keep[i]];
+      ns[i] = o->noff_h[keep[i]];
+      kp[i] = keep[i];
     }
-    AMSC_HIP(hipMemcpyAsync(s->Z + dn, o->Z + sn, sizeof(double) * nn,
-                            hipMemcpyDeviceToDevice, st));
-    AMSC_HIP(hipMemcpyAsync(s->ctrl + i, o->ctrl + k, sizeof(Ctrl),
-                            hipMemcpyDeviceToDevice, st));
-    s->it_host[i] = o->it_host[k];
-    s->ended[i] = o->ended[k];
-    s->ms_counts[2 * i] = o->ms_counts[2 * k];
-    s->ms_counts[2 * i + 1] = o->ms_counts[2 * k + 1];
+    int *dtab = nullptr;
+    if (int rc = dalloc(*s, &dtab, tab.size())) return rc;
+    AMSC_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(int) * tab.size(),
+                            hipMemcpyHostToDevice, st));
+    const RepackMap mp{dtab, dtab + nkeep + 1, nkeep};
+    const RepackMap mn{dtab + 2 * nkeep + 1, dtab + 3 * nkeep + 2, nkeep};
+    const int np_keep = pd[nkeep], nn_keep = nd[nkeep];
+    int rc = AMSC_OK;
+    if (np_keep > 0)
+      rc = s->mixed ? launch(*s, "repack", repack_particles_kernel<float>,
+                             dim3(grid_for(np_keep, 256)), dim3(256),
+                             o->buf[o->cur], s->buf[0], mp)
+                    : launch(*s, "repack", repack_particles_kernel<double>,
+                             dim3(grid_for(np_keep, 256)), dim3(256),
+                             o->buf[o->cur], s->buf[0], mp);
+    if (!rc)
+      rc = s->mixed ? launch(*s, "repack", repack_nodes_kernel<float>,
+                             dim3(grid_for(nn_keep, 256)), dim3(256), o->nod,
+                             s->nod, (const double *)o->Z, s->Z, mn)
+                    : launch(*s, "repack", repack_nodes_kernel<double>,
+                             dim3(grid_for(nn_keep, 256)), dim3(256), o->nod,
+                             s->nod, (const double *)o->Z, s->Z, mn);
+    if (!rc)
+      rc = launch(*s, "repack", repack_ctrl_kernel, dim3(nkeep), dim3(64),
+                  (const Ctrl *)o->ctrl, s->ctrl, (const int *)(dtab + 4 * nkeep + 2));
+    if (rc) return rc;
+    for (int i = 0; i < nkeep; ++i) {
+      const int k = keep[i];
+      s->it_host[i] = o->it_host[k];
+      s->ended[i] = o->ended[k];
+      s->ms_counts[2 * i] = o->ms_counts[2 * k];
+      s->ms_counts[2 * i + 1] = o->ms_counts[2 * k + 1];
+    }
   }
   if (nadd > 0) {
     const size_t z0 = s->noff_h[nkeep];
@@ -1307,9 +1368,11 @@ long long bytes_estimate(int S, const amsc_params *prm, const int *np,
   const long long r = precision == AMSC_PREC_MIXED ? 4 : 8;
   const long long per_p = 2 * (4 * 8 + (PF_COUNT - 4) * r + 8) + 7 * 4 + 4;
   const long long per_n = 2 * r + (NF_COUNT - 2) * 8 + 8;
-  return cap * per_p + (cap / 64 + 1) * 4 + nnt * per_n + nct * 8 +
+  // + upload staging: 3 FP64 values per initial particle
+  return cap * per_p + npt * 24 + (cap / 64 + 1) * 4 + nnt * per_n + nct * 8 +
          ntl * (6 * 8 + 4 * kTileCfgs) +
-         S * (long long)(sizeof(Ctrl) + kTileCfgs * sizeof(Scen) + 40) + 4096;
+         S * (long long)(sizeof(Ctrl) + kTileCfgs * sizeof(Scen) + 40) +
+         (1 << 16);  // allocations are rounded up to 256 bytes
 }
 
 }  // namespace
@@ -1501,6 +1564,7 @@ int amsc_ens_set_field(amsc_ens *e, int scen, const char *name,
 int amsc_ens_conservation(amsc_ens *e, int scen, double out[3]) {
   if (!e || !out || scen < 0 || scen >= e->S)
     return fail(AMSC_ERR_INVALID, "bad arguments");
+  if (int rc = reduce_conservation(*e)) return rc;
   if (int rc = read_ctrl(*e)) return rc;
   out[0] = e->ctrl_host[scen].cons[0];
   out[1] = e->ctrl_host[scen].cons[1];
@@ -1540,11 +1604,18 @@ int amsc_ens_repack(amsc_ens *e, int nkeep, const int *keep, int nadd,
                         vpx, vpy, mom_px, mom_py, hp, Vp, Z, capacity);
   if (rc) {
     const std::string msg = last_error_cstr();
+    if (s->st == e->st) s->st = nullptr;  // the stream stays with e
     sim_free(s);
     return fail(rc, msg);
   }
+  e->st = nullptr;  // now owned by s
   sim_free(e);
   *out = s;
+  return AMSC_OK;
+}
+
+int amsc_release_cache(void) {
+  cache_release();
   return AMSC_OK;
 }
 

@@ -624,14 +624,15 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   }
 }
 
-// err_mass, err_mom of every scenario from its tiles' partials, one
+// err_mass, err_mom of every scenario from its tiles' partials of the last
+// P2G it ran (tiles of an ended scenario no longer write them), one
 // workgroup per scenario, optim_par.cpp:416-425.
 __global__ void __launch_bounds__(256)
     conservation_kernel(const Scen *__restrict__ scen,
                         const double *__restrict__ partials, Ctrl *ctrls) {
   const Scen &sc = scen[blockIdx.x];
   Ctrl *ctrl = &ctrls[blockIdx.x];
-  if (!ctrl->active) return;
+  if (ctrl->it == 0) return;  // no step taken yet
   __shared__ double s[6][4];
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < sc.ntiles; i += 256)
@@ -1132,6 +1133,63 @@ __global__ void ms_execute_kernel(ExecArgs a) {
     out.label[o] = -1;
     out.level[o] = in.level[i] + 1;
   }
+}
+
+// ------------------------------------------------------------- repack
+// A served batch rebuilt with some scenarios kept: entry i of kept scenario
+// j (found by binary search over the new offsets) comes from old entry
+// src_off[j] + (i - dst_off[j]).
+struct RepackMap {
+  const int *dst_off;  // [nkeep+1] new offsets of the kept scenarios
+  const int *src_off;  // [nkeep] their old offsets
+  int nkeep;
+};
+
+__device__ __forceinline__ int repack_src(const RepackMap &m, int i) {
+  int lo = 0, hi = m.nkeep - 1;  // last j with dst_off[j] <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.dst_off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return m.src_off[lo] + (i - m.dst_off[lo]);
+}
+
+template <typename R>
+__global__ void repack_particles_kernel(PPtr in, PPtr out, RepackMap m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.dst_off[m.nkeep]) return;
+  const int j = repack_src(m, i);
+#pragma unroll 4
+  for (int k = 0; k < PF_COUNT; ++k) {
+    if (pf_wide(k)) fld<double>(out, k)[i] = fld<double>(in, k)[j];
+    else fld<R>(out, k)[i] = fld<R>(in, k)[j];
+  }
+  out.label[i] = in.label[j];
+  out.level[i] = in.level[j];
+}
+
+template <typename R>
+__global__ void repack_nodes_kernel(NPtr in, NPtr out, const double *zin,
+                                    double *zout, RepackMap m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.dst_off[m.nkeep]) return;
+  const int j = repack_src(m, i);
+#pragma unroll 4
+  for (int k = 0; k < NF_COUNT; ++k) {
+    if (nf_narrow(k)) static_cast<R *>(out.f[k])[i] = static_cast<const R *>(in.f[k])[j];
+    else static_cast<double *>(out.f[k])[i] = static_cast<const double *>(in.f[k])[j];
+  }
+  zout[i] = zin[j];
+}
+
+// Control blocks of the kept scenarios, one workgroup per scenario.
+__global__ void repack_ctrl_kernel(const Ctrl *in, Ctrl *out, const int *keep) {
+  constexpr int W = sizeof(Ctrl) / sizeof(unsigned long long);
+  const unsigned long long *src =
+      reinterpret_cast<const unsigned long long *>(in + keep[blockIdx.x]);
+  unsigned long long *dst = reinterpret_cast<unsigned long long *>(out + blockIdx.x);
+  for (int w = threadIdx.x; w < W; w += blockDim.x) dst[w] = src[w];
 }
 
 // ------------------------------------------------------------- fills

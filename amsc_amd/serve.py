@@ -5,10 +5,12 @@ simulated end time) arrive as a Poisson process.  Two forms serve them:
 
 * batched: every admitted request is a scenario of one device-resident
   `Ensemble`; the batch advances `chunk` steps at a time; finished scenarios
-  leave and arrived ones join through a device-side repack (the state of a
-  running scenario never leaves the GPU).  Admission keeps the device bytes
-  of the old and the new batch together (both exist during a repack) within
-  the HBM budget;
+  leave (at the next join, or once they hold half the batch) and arrived
+  ones join through a device-side repack (the state of a running scenario
+  never leaves the GPU).  Admission keeps the device bytes of the old and
+  the new batch together (both exist during a repack) within the HBM
+  budget.  Steps are launched eagerly by default: a HIP graph would have to
+  be recorded again after every repack;
 * sequential: one `Simulation` per request, first come first served (the
   single-scenario driver loop).
 
@@ -81,7 +83,7 @@ class BatchedServer:
     """Batched form over amsc_ens_* (see module docstring)."""
 
     def __init__(self, precision: str = "mixed", budget_bytes: int | None = None,
-                 chunk: int = 10, graph: bool = True, capacity_factor: float = 4.0,
+                 chunk: int = 10, graph: bool = False, capacity_factor: float = 4.0,
                  collect=()):
         self.precision = precision
         self.budget = budget_bytes
@@ -112,6 +114,7 @@ class BatchedServer:
         ens = None
         t0 = clock()
         repacks = 0
+        repack_s = 0.0
         while pending or active:
             now = clock() - t0
             # admission, first come first served, within the budget
@@ -125,8 +128,17 @@ class BatchedServer:
                 r.admitted = now
                 joining.append(r)
             if joining:
-                keep = np.arange(len(active), dtype=np.int32)
-                newreq = active + joining
+                t_r = clock()
+                if ens is not None:
+                    st0 = ens.status()
+                    keep = np.array([i for i in range(len(active)) if st0["active"][i]],
+                                    np.int32)
+                    for i in range(len(active)):
+                        if not st0["active"][i] and self.collect:
+                            active[i].result = {f: ens.get(i, f) for f in self.collect}
+                else:
+                    keep = np.zeros(0, np.int32)
+                newreq = [active[i] for i in keep] + joining
                 cap = self._capacity(newreq)
                 P = (native.AmscParams * len(joining))(
                     *[native.AmscParams.from_dict(r.prm) for r in joining])
@@ -152,6 +164,7 @@ class BatchedServer:
                 active = newreq
                 ens.nscen = len(active)
                 ens.prms = [r.prm for r in active]
+                repack_s += clock() - t_r
             if not active:
                 if pending:
                     sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.05)))
@@ -166,20 +179,24 @@ class BatchedServer:
                 r.cur_np = int(st["np"][i])
                 if st["active"][i]:
                     still.append(i)
-                else:
+                elif math.isnan(r.done):
                     r.done = now
-            if len(still) != len(active):
+                    if log:
+                        log(f"[serve] request {r.rid} done: {r.steps} steps, "
+                            f"{r.done - r.arrival:.3f} s")
+            # finished scenarios leave at the next join, or once they hold
+            # half the batch's particles (a repack rebuilds the batch)
+            fin_np = sum(r.live_np for i, r in enumerate(active) if not st["active"][i])
+            arrived = bool(pending) and pending[0].arrival <= now
+            leave = still and (arrived or 2 * fin_np > sum(r.live_np for r in active))
+            if not still or (len(still) != len(active) and leave):
                 keep_set = set(still)
-                finished = [r for i, r in enumerate(active) if i not in keep_set]
                 if self.collect:
                     for i, r in enumerate(active):
                         if i not in keep_set:
                             r.result = {f: ens.get(i, f) for f in self.collect}
-                if log:
-                    for r in finished:
-                        log(f"[serve] request {r.rid} done: {r.steps} steps, "
-                            f"{r.done - r.arrival:.3f} s")
                 if still:
+                    t_r = clock()
                     keep = np.array(still, np.int32)
                     h = C.c_void_p()
                     native.check(L.amsc_ens_repack(
@@ -187,6 +204,7 @@ class BatchedServer:
                         self._capacity([active[i] for i in still]), C.byref(h)))
                     ens._h = h
                     repacks += 1
+                    repack_s += clock() - t_r
                     active = [active[i] for i in still]
                     ens.nscen = len(active)
                     ens.prms = [r.prm for r in active]
@@ -200,7 +218,7 @@ class BatchedServer:
         tts = [r.done - r.arrival for r in requests]
         return {"form": f"batched-{self.precision}", "requests": len(requests),
                 "wall_s": wall, "scen_per_s": len(requests) / wall,
-                "repacks": repacks, **_pcts(tts)}
+                "repacks": repacks, "repack_s": repack_s, **_pcts(tts)}
 
 
 class SequentialServer:

@@ -351,6 +351,9 @@ int amsc_ens_repack(amsc_ens *e, int nkeep, const int *keep, int nadd,
                     const double *mom_px, const double *mom_py,
                     const double *hp, const double *Vp, const double *Z,
                     long long capacity, amsc_ens **out);
+/* Device memory of destroyed simulations is cached for the next ones of
+ * the process (at most 32 GiB); this returns it to the device. */
+int amsc_release_cache(void);
 /* Device bytes amsc_ens_create would allocate (admission control); -1 on
  * bad arguments. */
 long long amsc_ens_bytes_estimate(int nscen, const amsc_params *params,
