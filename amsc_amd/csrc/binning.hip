@@ -104,15 +104,23 @@ __global__ void colouring_kernel(int nrows, int ncols, int *ccidx, int o1,
 
 }  // namespace
 
-int scan_reserve(ScanWork &w, int n) {
-  size_t bytes = 0;
+int scan_bytes(int n, size_t *bytes) {
+  *bytes = 0;
   const hipError_t e = rocprim::inclusive_scan(
-      nullptr, bytes, static_cast<const int *>(nullptr),
+      nullptr, *bytes, static_cast<const int *>(nullptr),
       static_cast<int *>(nullptr), (size_t)(n > 0 ? n : 1),
       rocprim::plus<int>());
   if (e != hipSuccess)
     return fail(AMSC_ERR_HIP, std::string("scan sizing: ") + hipGetErrorString(e));
+  return AMSC_OK;
+}
+
+int scan_reserve(ScanWork &w, int n) {
+  size_t bytes = 0;
+  if (int rc = scan_bytes(n, &bytes)) return rc;
   if (w.bytes >= bytes && w.tmp) return AMSC_OK;
+  if (!w.owned)
+    return fail(AMSC_ERR_INVALID, "scan workspace too small for the scan");
   if (w.tmp) (void)hipFree(w.tmp);
   w.tmp = nullptr;
   w.bytes = 0;

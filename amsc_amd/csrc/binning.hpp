@@ -9,7 +9,11 @@ namespace amsc {
 struct ScanWork {
   void *tmp = nullptr;
   size_t bytes = 0;
+  bool owned = true;  // false: tmp belongs to the caller, never reallocated
 };
+
+// Workspace bytes of a scan of n values.
+int scan_bytes(int n, size_t *bytes);
 
 // out[1..n] = inclusive prefix of in[0..n); out[0] is not written and must
 // already hold 0, so out[0..n] is the exclusive prefix with the total in

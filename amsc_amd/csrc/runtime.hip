@@ -39,4 +39,37 @@ int amsc_synchronize(void) {
   return AMSC_OK;
 }
 
+int amsc_device_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return amsc::fail(AMSC_ERR_INVALID, "NULL output pointer");
+  *ptr = nullptr;
+  if (hipMalloc(ptr, bytes > 0 ? bytes : 1) != hipSuccess)
+    return amsc::fail(AMSC_ERR_NOMEM, "device allocation failed");
+  return AMSC_OK;
+}
+
+int amsc_device_free(void *ptr) {
+  if (ptr) AMSC_HIP(hipFree(ptr));
+  return AMSC_OK;
+}
+
+int amsc_copy_to_device(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return AMSC_OK;
+  AMSC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return AMSC_OK;
+}
+
+int amsc_copy_on_device(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return AMSC_OK;
+  AMSC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                          amsc::current_stream()));
+  return AMSC_OK;
+}
+
+int amsc_copy_to_host(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return AMSC_OK;
+  AMSC_HIP(hipDeviceSynchronize());
+  AMSC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return AMSC_OK;
+}
+
 }  // extern "C"

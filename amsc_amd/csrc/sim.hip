@@ -97,6 +97,7 @@ struct Sim {
   int ntiles[kTileCfgs] = {};
   Ctrl *ctrl = nullptr;
   Ctrl *ctrl_host = nullptr;  // pinned, S entries (headers only are read)
+  size_t ctrl_host_bytes = 0, host_ints_bytes = 0;
   int *poff = nullptr, *poff_next = nullptr, *wscen = nullptr;
   int *cell_off = nullptr, *err = nullptr;
   int *host_ints = nullptr;  // pinned staging, 4*S + S + 2 ints
@@ -236,6 +237,44 @@ void cache_release() {
   c.held = 0;
 }
 
+// Pinned host buffers, cached the same way (hipHostMalloc is slow).
+struct PinCache {
+  std::mutex m;
+  std::multimap<size_t, void *> blocks;
+};
+
+PinCache &pin_cache() {
+  static PinCache c;
+  return c;
+}
+
+int pinned_take(size_t b, void **p, size_t *got) {
+  {
+    PinCache &c = pin_cache();
+    std::lock_guard<std::mutex> g(c.m);
+    auto it = c.blocks.lower_bound(b);
+    if (it != c.blocks.end() && it->first <= 4 * b + 4096) {
+      *p = it->second;
+      *got = it->first;
+      c.blocks.erase(it);
+      return AMSC_OK;
+    }
+  }
+  AMSC_HIP(hipHostMalloc(p, b));
+  *got = b;
+  return AMSC_OK;
+}
+
+void pinned_give(void *p, size_t b) {
+  PinCache &c = pin_cache();
+  std::lock_guard<std::mutex> g(c.m);
+  if (c.blocks.size() >= 16) {
+    (void)hipHostFree(p);
+    return;
+  }
+  c.blocks.emplace(b, p);
+}
+
 template <typename T> int dalloc(Sim &s, T **p, size_t n) {
   const size_t b = (sizeof(T) * (n > 0 ? n : 1) + 255) & ~size_t(255);
   void *q = nullptr;
@@ -272,10 +311,8 @@ void sim_free(Sim *s) {
   if (s->st) (void)hipStreamSynchronize(s->st);
   for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.second.exec);
   for (auto &a : s->allocs) cache_give(a.first, a.second);
-  binning_free(s->bin);
-  if (s->ms_scan.tmp) (void)hipFree(s->ms_scan.tmp);
-  if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
-  if (s->host_ints) (void)hipHostFree(s->host_ints);
+  if (s->ctrl_host) pinned_give(s->ctrl_host, s->ctrl_host_bytes);
+  if (s->host_ints) pinned_give(s->host_ints, s->host_ints_bytes);
   for (auto e : s->event_pool) (void)hipEventDestroy(e);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
@@ -834,12 +871,28 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   if (int rc = dalloc(*s, &s->partials, 6 * (size_t)maxt)) return rc;
   for (int b = 0; b < 2; ++b)
     if (int rc = dalloc(*s, &s->p2g_out[b], 1)) return rc;
-  AMSC_HIP(hipHostMalloc(&s->ctrl_host, sizeof(Ctrl) * S));
-  AMSC_HIP(hipHostMalloc(&s->host_ints, sizeof(int) * (5 * S + 8)));
-  if (int rc = binning_alloc(s->bin, s->nc, 1)) return rc;
-  if (int rc = scan_reserve(s->bin.scan, s->nc)) return rc;
-  if (int rc = scan_reserve(s->ms_scan, s->cap)) return rc;
-  s->bytes += sizeof(int) * (2 * (size_t)s->nc + 1);
+  if (int rc = pinned_take(sizeof(Ctrl) * S, (void **)&s->ctrl_host,
+                           &s->ctrl_host_bytes))
+    return rc;
+  if (int rc = pinned_take(sizeof(int) * (5 * S + 8), (void **)&s->host_ints,
+                           &s->host_ints_bytes))
+    return rc;
+  // binning arrays and scan workspaces from the device cache
+  s->bin.nrows = s->nc;
+  s->bin.ncols = 1;
+  if (int rc = dalloc(*s, &s->bin.count, s->nc)) return rc;
+  if (int rc = dalloc(*s, &s->bin.cstart, (size_t)s->nc + 1)) return rc;
+  for (ScanWork *w : {&s->bin.scan, &s->ms_scan}) {
+    size_t b = 0;
+    if (int rc = scan_bytes(w == &s->ms_scan ? s->cap : s->nc, &b)) return rc;
+    char *q = nullptr;
+    if (int rc = dalloc(*s, &q, b)) return rc;
+    w->tmp = q;
+    w->bytes = b;
+    w->owned = false;
+  }
+  if (int rc = launch_fill_int(*s, s->bin.count, s->nc, 0)) return rc;
+  if (int rc = launch_fill_int(*s, s->bin.cstart, 1, 0)) return rc;
   for (int b = 0; b < 2; ++b) {
     P2GOut o;
     o.dzx = s->buf[b].f[PF_DZX];

@@ -141,6 +141,11 @@ def lib():
     L.amsc_sim_conservation.argtypes = [_vp, _dp]
     L.amsc_sim_ms_counts.argtypes = [_vp, _ip]
     L.amsc_sim_particle_steps.argtypes = [_vp, C.POINTER(C.c_longlong)]
+    L.amsc_device_alloc.argtypes = [C.c_size_t, C.POINTER(_vp)]
+    L.amsc_device_free.argtypes = [_vp]
+    L.amsc_copy_to_device.argtypes = [_vp, _vp, C.c_size_t]
+    L.amsc_copy_to_host.argtypes = [_vp, _vp, C.c_size_t]
+    L.amsc_copy_on_device.argtypes = [_vp, _vp, C.c_size_t]
     L.amsc_sim_set_graph.argtypes = [_vp, C.c_int]
     L.amsc_sim_set_tiling.argtypes = [_vp, C.c_int, C.c_int]
     L.amsc_sim_autotune.argtypes = [_vp, C.c_int, C.POINTER(C.c_int),
@@ -197,3 +202,35 @@ def ms_decide(prm: dict, np_, dt, ptrs, d_level, d_cstart, d_cptcls,
     check(lib().amsc_ms_decide(C.byref(P), np_, dt, *ptrs, d_level, d_cstart,
                                d_cptcls, d_action, d_partner, counts))
     return counts
+
+
+class DeviceArray:
+    """A numpy array's copy in device memory (library runtime), for callers
+    of the kernel-level entry points without another device allocator."""
+
+    def __init__(self, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        self.dtype, self.shape = a.dtype, a.shape
+        self.nbytes = a.nbytes
+        p = C.c_void_p()
+        check(lib().amsc_device_alloc(max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+        if self.nbytes:
+            check(lib().amsc_copy_to_device(self.ptr, a.ctypes.data, self.nbytes))
+
+    @classmethod
+    def zeros(cls, n, dtype):
+        return cls(np.zeros(n, dtype))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().amsc_copy_to_host(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().amsc_device_free(self.ptr)
+            self.ptr = None
+
+    __del__ = free

@@ -20,6 +20,8 @@
 #ifndef AMSC_MPM_H
 #define AMSC_MPM_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -35,6 +37,16 @@ const char *amsc_last_error(void);
 /* hipStream_t for subsequent launches (NULL = null stream). */
 int amsc_set_stream(void *stream);
 int amsc_synchronize(void);
+
+/* Device memory on the library's own HIP runtime (for callers that have no
+ * other allocator, e.g. Python tests): allocate, free, and synchronous
+ * copies (the copy to the host waits for the device first; the
+ * device-to-device copy is stream-ordered). */
+int amsc_device_alloc(size_t bytes, void **ptr);
+int amsc_device_free(void *ptr);
+int amsc_copy_to_device(void *dst, const void *src, size_t bytes);
+int amsc_copy_on_device(void *dst, const void *src, size_t bytes); /* async */
+int amsc_copy_to_host(void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------ 1. */
 /* Same parameters and results as gpu_kernels:: in the reference

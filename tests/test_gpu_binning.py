@@ -10,22 +10,19 @@ pytestmark = pytest.mark.gpu
 
 
 def device_bin(x, y, nrows, ncols, hx, hy):
-    import torch
     from amsc_amd import native
+    D = native.DeviceArray
     n = len(x)
-    tx = torch.from_numpy(np.ascontiguousarray(x, np.float64)).cuda()
-    ty = torch.from_numpy(np.ascontiguousarray(y, np.float64)).cuda()
-    p2g = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
-    cptcls = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
-    cstart = torch.zeros(nrows * ncols + 1, dtype=torch.int32, device="cuda")
-    ccidx = torch.zeros(max(nrows * ncols, 1), dtype=torch.int32, device="cuda")
+    tx, ty = D(x, np.float64), D(y, np.float64)
+    p2g = D.zeros(max(n, 1), np.int32)
+    cptcls = D.zeros(max(n, 1), np.int32)
+    cstart = D.zeros(nrows * ncols + 1, np.int32)
+    ccidx = D.zeros(max(nrows * ncols, 1), np.int32)
     print(f"[bin] n={n} grid={nrows}x{ncols}", flush=True)
-    coloff = native.bin_particles(n, nrows, ncols, hx, hy, tx.data_ptr(),
-                                  ty.data_ptr(), p2g.data_ptr(), cstart.data_ptr(),
-                                  cptcls.data_ptr(), ccidx.data_ptr())
-    torch.cuda.synchronize()
-    return (p2g.cpu().numpy()[:n], cstart.cpu().numpy(), cptcls.cpu().numpy()[:n],
-            ccidx.cpu().numpy()[:nrows * ncols], coloff)
+    coloff = native.bin_particles(n, nrows, ncols, hx, hy, tx.ptr, ty.ptr,
+                                  p2g.ptr, cstart.ptr, cptcls.ptr, ccidx.ptr)
+    return (p2g.numpy()[:n], cstart.numpy(), cptcls.numpy()[:n],
+            ccidx.numpy()[:nrows * ncols], coloff)
 
 
 def check(x, y, nrows, ncols, hx, hy):

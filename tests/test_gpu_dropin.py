@@ -19,7 +19,6 @@ def bits(a):
 
 
 def run_device(name, case):
-    import torch
     from amsc_amd import native
     args, outs, keep = [], {}, []
     for a, k in zip(ARGS[name], kinds(name)):
@@ -31,24 +30,24 @@ def run_device(name, case):
         elif k == "H":
             args.append(np.ascontiguousarray(v, np.int32).copy())
         elif k == "I":
-            t = torch.from_numpy(np.ascontiguousarray(v, np.int32).copy()).cuda()
+            t = native.DeviceArray(v, np.int32)
             keep.append(t)
-            args.append(t.data_ptr())
+            args.append(t.ptr)
         elif k in "PO":
             if a.endswith("_out"):
                 args.append(None)
                 continue
-            t = torch.from_numpy(np.ascontiguousarray(v, np.float64).copy()).cuda()
+            t = native.DeviceArray(v, np.float64)
             keep.append(t)
-            args.append(t.data_ptr())
+            args.append(t.ptr)
             if k == "O":
                 outs[a] = t
         else:
             args.append(None)
     print(f"[dropin] {name}", flush=True)
     native.dropin(name)(*args)
-    torch.cuda.synchronize()
-    return {k: t.cpu().numpy() for k, t in outs.items()}
+    native.check(native.lib().amsc_synchronize())
+    return {k: t.numpy() for k, t in outs.items()}
 
 
 @pytest.mark.parametrize("name", sorted(ARGS))

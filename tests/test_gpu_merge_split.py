@@ -35,26 +35,25 @@ def snapshot(seed, n=6000, nrows=30, ncols=40, h=1.0):
 
 
 def device_decide(prm, s, dt):
-    import torch
     from amsc_amd import native
+    D = native.DeviceArray
     n = len(s["x"])
     keep = []
 
-    def dev(a, dt_=torch.float64):
-        t = torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt_)
+    def dev(a, dtype=np.float64):
+        t = D(a, dtype)
         keep.append(t)
-        return t.data_ptr()
+        return t.ptr
     _, cstart, cptcls, _, _ = oracle.bin_particles(s["x"], s["y"], prm["nrows"],
                                                    prm["ncols"], prm["hx"], prm["hy"])
     ptrs = [dev(s[k]) for k in ("x", "y", "Ap", "hp", "vpx_dx", "vpx_dy",
                                 "vpy_dx", "vpy_dy", "vpx", "vpy")]
-    act = torch.zeros(n, dtype=torch.int32, device="cuda")
-    part = torch.zeros(n, dtype=torch.int32, device="cuda")
-    counts = native.ms_decide(prm, n, dt, ptrs, dev(s["level"], torch.int32),
-                              dev(cstart, torch.int32), dev(cptcls, torch.int32),
-                              act.data_ptr(), part.data_ptr())
-    torch.cuda.synchronize()
-    return act.cpu().numpy(), part.cpu().numpy(), counts
+    act = D.zeros(n, np.int32)
+    part = D.zeros(n, np.int32)
+    counts = native.ms_decide(prm, n, dt, ptrs, dev(s["level"], np.int32),
+                              dev(cstart, np.int32), dev(cptcls, np.int32),
+                              act.ptr, part.ptr)
+    return act.numpy(), part.numpy(), counts
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
