@@ -68,10 +68,18 @@ def make_requests(datas, arrivals, max_steps: int = 0):
 
 def _pcts(tts):
     a = np.asarray(tts, np.float64)
+    a = a[np.isfinite(a)]  # requests finished within the run's budget
     if a.size == 0:
         return {"p50_s": None, "p99_s": None, "mean_s": None}
     return {"p50_s": float(np.percentile(a, 50)), "p99_s": float(np.percentile(a, 99)),
             "mean_s": float(a.mean())}
+
+
+INT_FIELDS = ("label", "level")
+
+
+def _grab(get, get_int, fields):
+    return {f: (get_int(f) if f in INT_FIELDS else get(f)) for f in fields}
 
 
 def _cat(reqs, key):
@@ -92,6 +100,9 @@ class BatchedServer:
         self.capacity_factor = capacity_factor
         self.collect = tuple(collect)  # fields kept from each finished request
 
+    def _grab(self, ens, i):
+        return _grab(lambda f: ens.get(i, f), lambda f: ens.get_int(i, f), self.collect)
+
     def _capacity(self, reqs):
         # merge/split may split particles twice (ms_min_level = -2): room for
         # 4x the present count by default
@@ -104,7 +115,11 @@ class BatchedServer:
         return Ensemble.bytes_estimate([r.prm for r in reqs], [r.live_np for r in reqs],
                                        self.precision, self._capacity(reqs))
 
-    def run(self, requests, clock=time.perf_counter, sleep=time.sleep, log=None):
+    def run(self, requests, clock=time.perf_counter, sleep=time.sleep, log=None,
+            budget_s=math.inf, progress_s=10.0):
+        """Serve `requests`; stop after budget_s of wall clock (requests not
+        finished by then keep done = nan and are counted as unfinished);
+        log a progress line every progress_s."""
         import ctypes as C
         from . import native
         from .simulation import Ensemble, PRECISION
@@ -115,8 +130,14 @@ class BatchedServer:
         t0 = clock()
         repacks = 0
         repack_s = 0.0
+        last_log = 0.0
         while pending or active:
             now = clock() - t0
+            if now > budget_s:
+                if log:
+                    log(f"[serve] budget of {budget_s:.0f} s reached: "
+                        f"{len(active)} running, {len(pending)} waiting")
+                break
             # admission, first come first served, within the budget
             joining = []
             while pending and pending[0].arrival <= now:
@@ -135,7 +156,7 @@ class BatchedServer:
                                     np.int32)
                     for i in range(len(active)):
                         if not st0["active"][i] and self.collect:
-                            active[i].result = {f: ens.get(i, f) for f in self.collect}
+                            active[i].result = self._grab(ens, i)
                 else:
                     keep = np.zeros(0, np.int32)
                 newreq = [active[i] for i in keep] + joining
@@ -173,6 +194,12 @@ class BatchedServer:
             native.check(L.amsc_sim_step(ens._h, self.chunk, C.byref(done)))
             st = ens.status()
             now = clock() - t0
+            if log and now - last_log >= progress_s:
+                last_log = now
+                log(f"[serve] {now:.1f} s: {int(st['active'].sum())} running "
+                    f"(steps {int(st['it'].min())}..{int(st['it'].max())}, "
+                    f"sim time {float(st['t'].min()):.2f}..{float(st['t'].max()):.2f} s, "
+                    f"{int(st['np'].sum())} particles), {len(pending)} waiting")
             still = []
             for i, r in enumerate(active):
                 r.steps = int(st["it"][i])
@@ -194,7 +221,7 @@ class BatchedServer:
                 if self.collect:
                     for i, r in enumerate(active):
                         if i not in keep_set:
-                            r.result = {f: ens.get(i, f) for f in self.collect}
+                            r.result = self._grab(ens, i)
                 if still:
                     t_r = clock()
                     keep = np.array(still, np.int32)
@@ -216,34 +243,54 @@ class BatchedServer:
         if ens is not None:
             ens.close()
         tts = [r.done - r.arrival for r in requests]
+        fin = int(np.isfinite(tts).sum())
         return {"form": f"batched-{self.precision}", "requests": len(requests),
-                "wall_s": wall, "scen_per_s": len(requests) / wall,
+                "finished": fin, "wall_s": wall, "scen_per_s": fin / wall,
                 "repacks": repacks, "repack_s": repack_s, **_pcts(tts)}
 
 
 class SequentialServer:
     """One `Simulation` (FP64) per request, first come first served."""
 
-    def __init__(self, chunk: int = 100):
+    def __init__(self, chunk: int = 100, collect=()):
         self.chunk = chunk
+        self.collect = tuple(collect)  # fields kept from each finished request
 
-    def run(self, requests, clock=time.perf_counter, sleep=time.sleep, log=None):
+    def run(self, requests, clock=time.perf_counter, sleep=time.sleep, log=None,
+            budget_s=math.inf, progress_s=10.0):
+        """As BatchedServer.run."""
         from .simulation import Simulation
         pending = sorted(requests, key=lambda r: r.arrival)
         t0 = clock()
+        last_log = 0.0
         for r in pending:
             now = clock() - t0
+            if now > budget_s:
+                break
             if r.arrival > now:
                 sleep(r.arrival - now)
             r.admitted = clock() - t0
             s = Simulation(r.prm, r.particles, r.Z, capacity=4 * max(r.np, 1))
-            r.steps = s.run(self.chunk)
-            r.done = clock() - t0
+            while clock() - t0 <= budget_s:
+                k = s.step(self.chunk)
+                r.steps += k
+                now = clock() - t0
+                if k < self.chunk:
+                    r.done = now
+                    break
+                if log and now - last_log >= progress_s:
+                    last_log = now
+                    it, t, dt = s.state()
+                    log(f"[serve] {now:.1f} s: request {r.rid} at step {it}, "
+                        f"sim time {t:.2f} s, dt {dt:.2e}, {s.num_particles} particles")
+            if self.collect and not math.isnan(r.done):
+                r.result = _grab(s.get, s.get_int, self.collect)
             s.close()
-            if log:
+            if log and not math.isnan(r.done):
                 log(f"[serve] request {r.rid} done: {r.steps} steps, "
                     f"{r.done - r.arrival:.3f} s")
         wall = clock() - t0
         tts = [r.done - r.arrival for r in requests]
-        return {"form": "sequential-fp64", "requests": len(requests), "wall_s": wall,
-                "scen_per_s": len(requests) / wall, **_pcts(tts)}
+        fin = int(np.isfinite(tts).sum())
+        return {"form": "sequential-fp64", "requests": len(requests), "finished": fin,
+                "wall_s": wall, "scen_per_s": fin / wall, **_pcts(tts)}

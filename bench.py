@@ -21,12 +21,14 @@ The JSON line reports:
   ensemble       256 hazard scenarios (side ~ logU[64,1024] cells, particles
                  ~ logU[1e4,1e6] at 4 per cell, seeded topography and
                  rheology) advanced 100 steps each: batched mixed (one
-                 launch per phase, HIP graphs, autotuned tiles) vs 256
-                 sequential plain FP64 runs, in scenarios/s, with the
-                 per-field error of mixed against FP64;
+                 launch per phase, HIP graphs, autotuned tiles) and batched
+                 FP64 vs 256 sequential plain FP64 runs, in scenarios/s, with
+                 the per-field error of mixed against FP64;
   served         (--served) Poisson arrivals at 5, 20 and 80 requests/s and
                  a saturated queue, p50/p99 time-to-solution and
-                 scenarios/s, batched mixed vs sequential FP64;
+                 scenarios/s: batched mixed and batched FP64 vs sequential
+                 FP64, with the per-field error of each request against its
+                 sequential FP64 result;
   cpu_baseline   the FP64 CPU oracle on a bounded sample of the headline
                  workload (1 thread).
 
@@ -107,29 +109,51 @@ def cpu_baseline(d, steps, warm=2):
     return ps / dt, dt, o.np
 
 
-def gridded_depth(x, y, Vp, prm):
-    """Depth per cell: particle volume in the cell over the cell area."""
-    nr, nc = prm["nrows"], prm["ncols"]
-    c = np.clip(np.floor(x / prm["hx"]).astype(np.int64), 0, nc - 1)
-    r = np.clip(np.floor(y / prm["hy"]).astype(np.int64), 0, nr - 1)
-    return np.bincount(r + c * nr, weights=Vp, minlength=nr * nc) / (prm["hx"] * prm["hy"])
+def nodal_depth(x, y, Vp, prm):
+    """Depth field on the nodes: particle volume spread over the cell's four
+    nodes with the bilinear P2G weights, over the cell area.  Unlike a
+    per-cell volume count it is continuous in the particle positions, so a
+    particle a round-off away from a cell edge changes it by round-off."""
+    nr, nc, hx, hy = prm["nrows"], prm["ncols"], prm["hx"], prm["hy"]
+    c = np.clip(np.floor(x / hx).astype(np.int64), 0, nc - 1)
+    r = np.clip(np.floor(y / hy).astype(np.int64), 0, nr - 1)
+    fx = np.clip(x / hx - c, 0.0, 1.0)
+    fy = np.clip(y / hy - r, 0.0, 1.0)
+    nn = (nr + 1) * (nc + 1)
+    out = np.zeros(nn)
+    for dc, dr, w in ((0, 0, (1 - fx) * (1 - fy)), (1, 0, fx * (1 - fy)),
+                      (0, 1, (1 - fx) * fy), (1, 1, fx * fy)):
+        out += np.bincount((r + dr) + (c + dc) * (nr + 1), weights=w * Vp, minlength=nn)
+    return out / (hx * hy)
 
 
 def field_errors(a, b, prm):
-    """Error of run a against run b (both Simulation-like, scenario 0):
-    gridded depth relative to the maximum depth; per particle depth and
-    velocity when the particle sets match."""
-    ga = gridded_depth(a["x"], a["y"], a["Vp"], prm)
-    gb = gridded_depth(b["x"], b["y"], b["Vp"], prm)
-    out = {"grid_depth_max": float(np.abs(ga - gb).max() / max(gb.max(), 1e-30)),
-           "grid_depth_median_wet": float(np.median(np.abs(ga - gb)[gb > 0]) /
-                                          max(gb.max(), 1e-30)) if (gb > 0).any() else 0.0}
-    if len(a["hp"]) == len(b["hp"]):
+    """Error of run a against run b (dicts from grab()).  Per particle, over
+    the particles present in both runs, matched by label (merge/split gives
+    new particles label -1 and may decide differently in the two runs; the
+    reference's own CPU/GPU comparison matches by label the same way):
+    depth relative to b's maximum depth (max, p99, median, fraction within
+    1e-4) and velocity relative to sqrt(g hmax).  Field: the nodal depth
+    (nodal_depth), max and relative L2 error."""
+    ga = nodal_depth(a["x"], a["y"], a["Vp"], prm)
+    gb = nodal_depth(b["x"], b["y"], b["Vp"], prm)
+    gmax = max(gb.max(), 1e-30)
+    out = {"node_depth_max": float(np.abs(ga - gb).max() / gmax),
+           "node_depth_rel_l2": float(np.linalg.norm(ga - gb) /
+                                      max(np.linalg.norm(gb), 1e-30))}
+    la, lb = a["label"], b["label"]
+    common, ia, ib = np.intersect1d(la[la >= 0], lb[lb >= 0], return_indices=True)
+    ia = np.flatnonzero(la >= 0)[ia]
+    ib = np.flatnonzero(lb >= 0)[ib]
+    out["matched_frac"] = float(len(common) / max(len(lb), 1))
+    if len(common):
         hmax = max(np.abs(b["hp"]).max(), 1e-30)
-        cw = np.sqrt(9.81 * hmax)
-        dh = np.abs(a["hp"] - b["hp"]) / hmax
-        dv = np.hypot(a["vpx"] - b["vpx"], a["vpy"] - b["vpy"]) / cw
-        out.update({"hp_max": float(dh.max()), "hp_median": float(np.median(dh)),
+        dh = np.abs(a["hp"][ia] - b["hp"][ib]) / hmax
+        dv = np.hypot(a["vpx"][ia] - b["vpx"][ib], a["vpy"][ia] - b["vpy"][ib]) / \
+            np.sqrt(9.81 * hmax)
+        out.update({"hp_max": float(dh.max()), "hp_p99": float(np.percentile(dh, 99)),
+                    "hp_median": float(np.median(dh)),
+                    "hp_frac_within_1e-4": float(np.mean(dh <= 1e-4)),
                     "v_max_over_sqrt_gh": float(dv.max()),
                     "v_median_over_sqrt_gh": float(np.median(dv))})
     return out
@@ -137,7 +161,9 @@ def field_errors(a, b, prm):
 
 def grab(sim, k=None):
     g = (lambda f: sim.get(f)) if k is None else (lambda f: sim.get(k, f))
-    return {f: g(f) for f in ("x", "y", "Vp", "hp", "vpx", "vpy")}
+    out = {f: g(f) for f in ("x", "y", "Vp", "hp", "vpx", "vpy")}
+    out["label"] = sim.get_int("label") if k is None else sim.get_int(k, "label")
+    return out
 
 
 def run_headline(args, d, precision):
@@ -224,9 +250,10 @@ def ensemble_bench(args):
     out["fp64_batched"] = {"scen_per_s": n / wall64, "wall_s": wall64,
                            "particle_steps_per_s": ps64 / wall64}
     log(f"[bench] ensemble fp64 batched: {wall64:.3f} s, {n / wall64:.1f} scen/s")
+    fp64_state = [grab(e, k) for k in range(n)]
     e.close()
     # sequential plain FP64 (creation not timed)
-    seq, tts, errs = 0.0, [], []
+    seq, tts, errs, same = 0.0, [], [], 0
     for k, c in enumerate(cases):
         s = Simulation(*c, capacity=3 * len(c[1]["x"]))
         t = time.perf_counter()
@@ -234,45 +261,117 @@ def ensemble_bench(args):
         dt = time.perf_counter() - t
         seq += dt
         tts.append(seq)
-        errs.append(field_errors(mixed_state[k], grab(s), c[0]))
+        ref = grab(s)
+        errs.append(field_errors(mixed_state[k], ref, c[0]))
+        same += all(np.array_equal(fp64_state[k][f], ref[f]) for f in ref)
         s.close()
     out["fp64_sequential"] = {"scen_per_s": n / seq, "wall_s": seq,
                               "p50_s": float(np.percentile(tts, 50)),
                               "p99_s": float(np.percentile(tts, 99))}
     out["mixed_batched_vs_fp64_sequential"] = (n / wall) / (n / seq)
+    out["fp64_batched_vs_fp64_sequential"] = (n / wall64) / (n / seq)
+    out["fp64_batched_bit_identical_scenarios"] = int(same)
     log(f"[bench] ensemble fp64 sequential: {seq:.3f} s, {n / seq:.1f} scen/s; "
-        f"ratio {out['mixed_batched_vs_fp64_sequential']:.2f}")
+        f"ratio mixed {out['mixed_batched_vs_fp64_sequential']:.2f}, fp64 batched "
+        f"{out['fp64_batched_vs_fp64_sequential']:.2f} ({same}/{n} bit-identical)")
+    out["mixed_error_vs_fp64"] = aggregate_errors(errs)
+    return out
+
+
+def aggregate_errors(errs):
+    """Per-scenario field_errors() dicts -> max, median and min over scenarios,
+    and how many scenarios keep every matched particle within 1e-4."""
     keys = sorted({k for e_ in errs for k in e_})
-    out["mixed_error_vs_fp64"] = {
-        k: {"max": float(np.max([e_[k] for e_ in errs if k in e_])),
-            "median": float(np.median([e_[k] for e_ in errs if k in e_])),
-            "scenarios": int(sum(k in e_ for e_ in errs))} for k in keys}
+    out = {k: {"max": float(np.max([e_[k] for e_ in errs if k in e_])),
+               "median": float(np.median([e_[k] for e_ in errs if k in e_])),
+               "min": float(np.min([e_[k] for e_ in errs if k in e_])),
+               "scenarios": int(sum(k in e_ for e_ in errs))} for k in keys}
+    out["scenarios_hp_max_within_1e-4"] = int(sum(e_.get("hp_max", 1.0) <= 1e-4
+                                                  for e_ in errs))
+    return out
+
+
+def still_water_bench(args):
+    """Lake at rest over the 4M-particle configuration's topography
+    (eq_level = 10 m, zero velocity, hp = 10 - Zp with Zp the bilinear Z at
+    the particle): drift of the free surface hp + Zp - 10, of hp, of the
+    positions, and the largest speed after the run, both precisions."""
+    from amsc_amd import scenarios
+    from amsc_amd.data import params_from_data, particles_from_data
+    from amsc_amd.simulation import Ensemble
+    d = scenarios.large_fractal(still_water=True, nsteps=args.still_steps)
+    prm = params_from_data(d)
+    p0 = particles_from_data(d)
+    nex, ney, hx, hy = int(d["Nex"]), int(d["Ney"]), prm["hx"], prm["hy"]
+    Zg = np.asarray(d["Z"], np.float64).reshape(nex + 1, ney + 1)
+
+    def zp(x, y):
+        c = np.minimum(np.floor(x / hx).astype(np.int64), nex - 1)
+        r = np.minimum(np.floor(y / hy).astype(np.int64), ney - 1)
+        xi_, eta = (x - c * hx) / hx, (y - r * hy) / hy
+        return ((1 - xi_) * (1 - eta) * Zg[c, r] + (1 - xi_) * eta * Zg[c, r + 1]
+                + xi_ * (1 - eta) * Zg[c + 1, r] + xi_ * eta * Zg[c + 1, r + 1])
+
+    out = {"particles": len(p0["x"]), "steps": args.still_steps, "eq_level": 10.0}
+    log(f"[bench] still water: {out['particles']} particles, {args.still_steps} steps")
+    for precision in ("fp64", "mixed"):
+        e = Ensemble([(prm, p0, d["Z"])], precision)
+        done = e.run(args.still_steps + 1)
+        x, y, hp = e.get(0, "x"), e.get(0, "y"), e.get(0, "hp")
+        vx, vy = e.get(0, "vpx"), e.get(0, "vpy")
+        out[precision] = {
+            "steps_run": int(done),
+            "free_surface_drift_max": float(np.abs(hp + zp(x, y) - 10.0).max()),
+            "hp_drift_max": float(np.abs(hp - p0["hp"]).max()),
+            "position_drift_max": float(max(np.abs(x - p0["x"]).max(),
+                                            np.abs(y - p0["y"]).max())),
+            "speed_max": float(np.hypot(vx, vy).max())}
+        e.close()
+        log(f"[bench] still water {precision}: {out[precision]}")
     return out
 
 
 def served_bench(args):
     from amsc_amd import scenarios, serve
+    FIELDS = ("x", "y", "Vp", "hp", "vpx", "vpy", "label")
     out = {}
     # the three arrival rates of the served configuration, then a saturated
     # queue (every request waiting at t = 0) that measures service capacity
-    for rate in (5.0, 20.0, 80.0, 1e6):
+    for rate in (float(r) for r in args.served_rates.split(",")):
         datas = [scenarios.ensemble_scenario(5000 + i, np_range=(1e4, 4e6))
                  for i in range(args.served_requests)]
         arr = serve.poisson_arrivals(rate, len(datas), seed=int(rate))
         res = {}
         log(f"[bench] served {rate}/s: {len(datas)} requests, "
             f"{sum(len(d['x']) for d in datas)} particles")
-        for form in ("mixed", "fp64"):
+        done = {}
+        for form in ("mixed", "fp64_batched", "fp64"):
             reqs = serve.make_requests(datas, arr, max_steps=args.served_max_steps)
-            if form == "mixed":
-                srv = serve.BatchedServer("mixed", budget_bytes=int(0.8 * HBM_BYTES))
+            if form == "fp64":
+                srv = serve.SequentialServer(collect=FIELDS)
             else:
-                srv = serve.SequentialServer()
-            r = srv.run(reqs, log=log)
+                srv = serve.BatchedServer("mixed" if form == "mixed" else "fp64",
+                                          budget_bytes=int(0.8 * HBM_BYTES),
+                                          collect=FIELDS)
+            r = srv.run(reqs, log=log, budget_s=args.served_budget)
             r["steps_mean"] = float(np.mean([q.steps for q in reqs]))
             res[form] = r
-            log(f"[bench] served {rate}/s {form}: {r['scen_per_s']:.2f} scen/s, "
-                f"p50 {r['p50_s']:.3f} s, p99 {r['p99_s']:.3f} s")
+            done[form] = reqs
+            log(f"[bench] served {rate}/s {form}: {r['finished']}/{len(reqs)} finished, "
+                f"{r['scen_per_s']:.2f} scen/s, p50 {r['p50_s']} s, p99 {r['p99_s']} s")
+        # requests finished by both forms
+        pairs = {f: [(q, p) for q, p in zip(done[f], done["fp64"])
+                     if q.result is not None and p.result is not None]
+                 for f in ("mixed", "fp64_batched")}
+        res["mixed"]["error_vs_fp64"] = aggregate_errors(
+            [field_errors(q.result, p.result, p.prm) for q, p in pairs["mixed"]])
+        res["fp64_batched"]["bit_identical_requests"] = int(sum(
+            all(np.array_equal(q.result[f], p.result[f]) for f in FIELDS)
+            for q, p in pairs["fp64_batched"]))
+        res["fp64_batched"]["compared_requests"] = len(pairs["fp64_batched"])
+        for form in ("mixed", "fp64_batched"):
+            res[form]["vs_fp64_sequential"] = (res[form]["scen_per_s"] /
+                                               max(res["fp64"]["scen_per_s"], 1e-30))
         out["saturated" if rate >= 1e6 else f"lambda_{int(rate)}"] = res
     return out
 
@@ -292,7 +391,14 @@ def main():
     ap.add_argument("--ens-scenarios", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--served", action="store_true")
+    ap.add_argument("--still-water", action="store_true",
+                    help="also run the lake-at-rest drift check")
+    ap.add_argument("--still-steps", type=int, default=1000)
     ap.add_argument("--served-requests", type=int, default=24)
+    ap.add_argument("--served-rates", default="5,20,80,1e6",
+                    help="arrival rates (requests/s); 1e6 = saturated queue")
+    ap.add_argument("--served-budget", type=float, default=300.0,
+                    help="wall-clock seconds per served form and rate")
     ap.add_argument("--served-max-steps", type=int, default=400,
                     help="step cap per request (0: run to the seeded end time)")
     ap.add_argument("--tiling", default="", metavar="V,B",
@@ -359,6 +465,7 @@ def main():
 
     ens = None if args.no_ensemble else ensemble_bench(args)
     served = served_bench(args) if args.served else None
+    still = still_water_bench(args) if args.still_water else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -397,6 +504,7 @@ def main():
         "mixed": mixed,
         "ensemble": ens,
         "served": served,
+        "still_water": still,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
