@@ -420,7 +420,13 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
         s_own[q - base] = (unsigned short)((s << 8) | j);
     }
   };
+  // Positions past the list are clamped to its last entry, so that every
+  // thread issues the same loads whether or not its position is in the
+  // list: the compiler can then count the loads in flight exactly and wait
+  // for just the ones a stage needs (a load behind a branch may or may not
+  // have been issued, which forces a wait for all of them).
   auto fetch_idx = [&](int q, P2GIdx &I) {
+    q = q < L ? q : L - 1;
     const int v = s_own[q % CH];
     I.s = v >> 8;
     I.j = v & 0xFF;
@@ -456,13 +462,19 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
       }
     }
     __syncthreads();
-    if (t < L) fetch_idx(t, I1);
+    fetch_idx(t, I1);
     __syncthreads();
     build_own(CH);
     __syncthreads();
-    if (CH + t < L) fetch_idx(CH + t, I2);
-    if (t < L) p2g_fields<R>(a, I1.ip, W);
+    fetch_idx(CH + t, I2);
+    p2g_fields<R>(a, I1.ip, W);
     __syncthreads();  // s_own is read above and rebuilt in the loop
+    // Let the loads above land before the loop: otherwise the loop header
+    // merges a path on which they are still in flight, and the compiler
+    // then waits for every load in flight at the first reuse of one of
+    // their registers, which falls inside the fold, i.e. it would wait for
+    // the next chunk's prefetch before folding this one.
+    __builtin_amdgcn_s_waitcnt(0);
 
     for (int base = 0; base < L; base += CH) {
       if (base + t < L) {
@@ -525,8 +537,8 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
       build_own(base + 2 * CH);
       __syncthreads();
       // in flight during the fold: ids of chunk k+2, fields of chunk k+1
-      if (base + 2 * CH + t < L) fetch_idx(base + 2 * CH + t, I3);
-      if (base + CH + t < L) p2g_fields<R>(a, I2.ip, W);
+      fetch_idx(base + 2 * CH + t, I3);
+      p2g_fields<R>(a, I2.ip, W);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int lo = tb[k] > base ? tb[k] : base;
