@@ -90,9 +90,13 @@ def workload(name):
     return d, desc
 
 
-def cpu_baseline(d, steps, warm=2):
+def cpu_baseline(d, steps, threads, warm=2):
+    """The FP64 oracle (optim_par restated; per-particle loops over
+    `threads` OpenMP threads, P2G scatters serial as in optim_par) on the
+    host: particle-steps/s over `steps` steps after `warm` steps."""
     from amsc_amd.data import params_from_data, particles_from_data
     from oracle import oracle
+    oracle.lib().or_set_threads(threads)
     prm = params_from_data(d)
     prm["max_steps"] = 0
     o = oracle.OracleSim(prm, particles_from_data(d), d["Z"])
@@ -469,15 +473,22 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"[bench] CPU oracle baseline: {args.cpu_steps} steps")
-        v, secs, npc = cpu_baseline(d, args.cpu_steps)
-        cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": 1,
+        # the host's share of cores (OMP_NUM_THREADS where set)
+        nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        host = platform.processor() or platform.machine()
+        log(f"[bench] CPU oracle baseline: {args.cpu_steps} steps, {nth} threads, then 1")
+        v, secs, npc = cpu_baseline(d, args.cpu_steps, nth)
+        v1, secs1, _ = cpu_baseline(d, max(args.cpu_steps // 2, 2), 1)
+        cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": nth,
                "kind": "port",
                "sample": f"{args.config} workload ({np0} particles), {args.cpu_steps} "
                          f"steps after 2 warm-up steps, FP64 oracle restating "
-                         f"optim_par, 1 thread, {secs:.1f} s, host "
-                         f"{platform.processor() or platform.machine()}"}
-        log(f"[bench] CPU: {v:.4g} particle-steps/s")
+                         f"optim_par (per-particle loops on {nth} OpenMP threads, "
+                         f"P2G scatters serial), {secs:.1f} s, host {host}",
+               "single_thread": {"value": round(v1, 1), "cores": 1,
+                                 "steps": max(args.cpu_steps // 2, 2),
+                                 "seconds": round(secs1, 2)}}
+        log(f"[bench] CPU: {v:.4g} particle-steps/s on {nth} threads, {v1:.4g} on 1")
 
     out = {
         "metric": METRIC, "value": round(r64["value"], 1), "unit": "particle-steps/s",

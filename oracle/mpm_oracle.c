@@ -14,6 +14,35 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Per-particle and per-node loops of the time step run in parallel when the
+ * library is built with OpenMP and the arrays are large (each iteration
+ * writes only its own entry, so results do not depend on the thread count);
+ * scatters, sums and binning stay serial, in the reference's order. */
+#ifdef _OPENMP
+#include <omp.h>
+#define OR_PAR_NP _Pragma("omp parallel for schedule(static) if (np > 16384)")
+#define OR_PAR_NN _Pragma("omp parallel for schedule(static) if (nn > 16384)")
+#else
+#define OR_PAR_NP
+#define OR_PAR_NN
+#endif
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int or_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 /* ===================================================================== */
 /* quadgrid layout helpers: main/gpu_kernels.cpp:7-59                     */
 /* node id r + c*(nrows+1); cell id r + c*nrows; local nodes               */
@@ -985,6 +1014,7 @@ void or_stress_optim_par(int np, const or_params *p, const double *vpx,
                          const double *vpx_dx, const double *vpx_dy,
                          const double *vpy_dx, const double *vpy_dy,
                          double *F11, double *F12, double *F21, double *F22) {
+  OR_PAR_NP
   for (int ip = 0; ip < np; ++ip) {
     const double A = 3. / 2., C = 65. / 32.;
     const double BING = p->bingham_on, mu = p->mu, tau_Y = p->tauy;
@@ -1093,6 +1123,7 @@ int or_sim_step(or_sim *s) {
   p2g_field(s, f[OR_MOM_PX], v[NV_MOMX]);
   p2g_field(s, f[OR_MOM_PY], v[NV_MOMY]);
   /* g2pd(Z), optim_par.cpp:373-374,387 */
+  OR_PAR_NP
   for (int ip = 0; ip < np; ++ip)
     g2pd_one(s, ip, s->Z, &f[OR_DZXP][ip], &f[OR_DZYP][ip]);
 
@@ -1119,6 +1150,7 @@ int or_sim_step(or_sim *s) {
   }
 
   /* (2) external forces, optim_par.cpp:447-470 */
+  OR_PAR_NP
   for (int ip = 0; ip < np; ++ip) {
     const double wb =
         f[OR_HP][ip] > 1e-10 ? 1.0 - (f[OR_H][ip] / f[OR_HP][ip]) : 1.0;
@@ -1131,6 +1163,7 @@ int or_sim_step(or_sim *s) {
   p2g_field(s, f[OR_FPY], v[NV_FPY]);
   p2g_field(s, f[OR_FRIC_PX], v[NV_FRX]);
   p2g_field(s, f[OR_FRIC_PY], v[NV_FRY]);
+  OR_PAR_NN
   for (int i = 0; i < nn; ++i) {
     v[NV_FEXTX][i] = v[NV_FPX][i] + v[NV_FRX][i];
     v[NV_FEXTY][i] = v[NV_FPY][i] + v[NV_FRY][i];
@@ -1138,6 +1171,7 @@ int or_sim_step(or_sim *s) {
   /* (3) internal forces and momentum balance, optim_par.cpp:476-485 */
   p2gd_field(s, f[OR_F11], f[OR_F12], f[OR_VP], v[NV_FINTX]);
   p2gd_field(s, f[OR_F21], f[OR_F22], f[OR_VP], v[NV_FINTY]);
+  OR_PAR_NN
   for (int i = 0; i < nn; ++i) {
     v[NV_FTOTX][i] = v[NV_FEXTX][i] - v[NV_FINTX][i];
     v[NV_FTOTY][i] = v[NV_FEXTY][i] - v[NV_FINTY][i];
@@ -1145,6 +1179,7 @@ int or_sim_step(or_sim *s) {
     v[NV_MOMY][i] = dt * v[NV_FTOTY][i] + v[NV_MOMY][i];
   }
   /* (4) nodal accelerations and velocities, optim_par.cpp:492-497 (D1) */
+  OR_PAR_NN
   for (int i = 0; i < nn; ++i) {
     const double m = v[NV_MV][i];
     v[NV_AVX][i] = m > 1.e-2 ? v[NV_FTOTX][i] / m : 0.0;
@@ -1176,6 +1211,7 @@ int or_sim_step(or_sim *s) {
     }
   }
   /* (6) G2P, PIC velocity, advection, clamp: optim_par.cpp:534-564 */
+  OR_PAR_NP
   for (int ip = 0; ip < np; ++ip) {
     f[OR_VPXL][ip] = g2p_one(s, ip, v[NV_VVXL]);
     f[OR_VPYL][ip] = g2p_one(s, ip, v[NV_VVYL]);
@@ -1184,6 +1220,7 @@ int or_sim_step(or_sim *s) {
   }
   {
     const double eps = 1e-10, Lx = p->ncols * p->hx, Ly = p->nrows * p->hy;
+    OR_PAR_NP
     for (int ip = 0; ip < np; ++ip) {
       f[OR_VPX][ip] = f[OR_VPXL][ip];
       f[OR_VPY][ip] = f[OR_VPYL][ip];
@@ -1198,6 +1235,7 @@ int or_sim_step(or_sim *s) {
   /* (7) g2pd of the updated nodal velocities at the new positions with the
    * step-start cells (D18), height update, floor, momentum, area:
    * optim_par.cpp:569-607 */
+  OR_PAR_NP
   for (int ip = 0; ip < np; ++ip) {
     g2pd_one(s, ip, v[NV_VVXL], &f[OR_VPX_DX][ip], &f[OR_VPX_DY][ip]);
     g2pd_one(s, ip, v[NV_VVYL], &f[OR_VPY_DX][ip], &f[OR_VPY_DY][ip]);
@@ -1213,6 +1251,7 @@ int or_sim_step(or_sim *s) {
   {
     const double fric_ang = p->phi * M_PI / 180.;
     const double tanf = tan(fric_ang);
+    OR_PAR_NP
     for (int ip = 0; ip < np; ++ip) {
       const double vx = f[OR_VPX][ip], vy = f[OR_VPY][ip], h = f[OR_HP][ip];
       const double nv = sqrt(vx * vx + vy * vy);
@@ -1235,6 +1274,7 @@ int or_sim_step(or_sim *s) {
                       f[OR_VPX_DX], f[OR_VPX_DY], f[OR_VPY_DX], f[OR_VPY_DY],
                       f[OR_F11], f[OR_F12], f[OR_F21], f[OR_F22]);
   /* g2p(Z) at the new positions, hpZ (output fields), optim_par.cpp:637-646 */
+  OR_PAR_NP
   for (int ip = 0; ip < np; ++ip) {
     f[OR_ZP][ip] = g2p_one(s, ip, s->Z);
     f[OR_HPZ][ip] = f[OR_HP][ip] + f[OR_ZP][ip];

@@ -163,6 +163,10 @@ enum {
 
 typedef struct or_sim or_sim;
 
+/* Threads of the per-particle loops (OpenMP builds; no effect otherwise). */
+void or_set_threads(int n);
+int or_max_threads(void);
+
 or_sim *or_sim_create(const or_params *p, int np, const double *x,
                       const double *y, const double *Mp, const double *Ap,
                       const double *vpx, const double *vpy,
