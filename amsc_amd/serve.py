@@ -252,9 +252,10 @@ class BatchedServer:
 class SequentialServer:
     """One `Simulation` (FP64) per request, first come first served."""
 
-    def __init__(self, chunk: int = 100, collect=()):
+    def __init__(self, chunk: int = 100, collect=(), graph: bool = False):
         self.chunk = chunk
         self.collect = tuple(collect)  # fields kept from each finished request
+        self.graph = graph
 
     def run(self, requests, clock=time.perf_counter, sleep=time.sleep, log=None,
             budget_s=math.inf, progress_s=10.0):
@@ -271,6 +272,8 @@ class SequentialServer:
                 sleep(r.arrival - now)
             r.admitted = clock() - t0
             s = Simulation(r.prm, r.particles, r.Z, capacity=4 * max(r.np, 1))
+            if self.graph:
+                s.set_graph(True)
             while clock() - t0 <= budget_s:
                 k = s.step(self.chunk)
                 r.steps += k

@@ -352,11 +352,12 @@ def served_bench(args):
         for form in ("mixed", "fp64_batched", "fp64"):
             reqs = serve.make_requests(datas, arr, max_steps=args.served_max_steps)
             if form == "fp64":
-                srv = serve.SequentialServer(collect=FIELDS)
+                srv = serve.SequentialServer(collect=FIELDS, graph=args.served_graph)
             else:
                 srv = serve.BatchedServer("mixed" if form == "mixed" else "fp64",
                                           budget_bytes=int(0.8 * HBM_BYTES),
-                                          collect=FIELDS)
+                                          chunk=args.served_chunk,
+                                          graph=args.served_graph, collect=FIELDS)
             r = srv.run(reqs, log=log, budget_s=args.served_budget)
             r["steps_mean"] = float(np.mean([q.steps for q in reqs]))
             res[form] = r
@@ -403,6 +404,10 @@ def main():
                     help="arrival rates (requests/s); 1e6 = saturated queue")
     ap.add_argument("--served-budget", type=float, default=300.0,
                     help="wall-clock seconds per served form and rate")
+    ap.add_argument("--served-chunk", type=int, default=10,
+                    help="steps per batched step call between joins/leaves")
+    ap.add_argument("--served-graph", action="store_true",
+                    help="replay steps from HIP graphs in every served form")
     ap.add_argument("--served-max-steps", type=int, default=400,
                     help="step cap per request (0: run to the seeded end time)")
     ap.add_argument("--tiling", default="", metavar="V,B",

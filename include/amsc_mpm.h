@@ -299,7 +299,7 @@ int amsc_sim_set_graph(amsc_sim *s, int on);
 /* P2G variant v in 0..5: tile shape v % 3 (0: 16x32 nodes, 1: 16x16,
  * 2: 8x32), tile order v / 3 (0: each XCD takes a contiguous range of
  * tiles, 1: linear); particle-kernel workgroup size 256 or 512.  Results do
- * not depend on either. */
+ * not depend on either.  Default: variant 5 (8x32, linear) and 512. */
 #define AMSC_P2G_VARIANTS 6
 int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block);
 /* Autotuner: time every P2G variant, then every workgroup size, over
