@@ -50,6 +50,31 @@ __global__ void place_kernel(int np, const int *__restrict__ np_dev,
 
 constexpr int kSortLocal = 32;
 
+// Heap sort of a[0..n) in place (cells above kSortLocal particles: O(n log n)
+// however many particles pile into one cell).
+__device__ void heap_sort(int *a, int n) {
+  auto sift = [a](int root, int end) {
+    while (2 * root + 1 <= end) {
+      const int child = 2 * root + 1;
+      int sw = root;
+      if (a[sw] < a[child]) sw = child;
+      if (child + 1 <= end && a[sw] < a[child + 1]) sw = child + 1;
+      if (sw == root) return;
+      const int t = a[root];
+      a[root] = a[sw];
+      a[sw] = t;
+      root = sw;
+    }
+  };
+  for (int start = (n - 2) / 2; start >= 0; --start) sift(start, n - 1);
+  for (int end = n - 1; end > 0; --end) {
+    const int t = a[0];
+    a[0] = a[end];
+    a[end] = t;
+    sift(0, end - 1);
+  }
+}
+
 __global__ void cell_sort_kernel(int nc, const int *__restrict__ cstart,
                                  int *cptcls, int *count,
                                  const int *__restrict__ active) {
@@ -78,15 +103,7 @@ __global__ void cell_sort_kernel(int nc, const int *__restrict__ cstart,
     }
     for (int i = 0; i < n; ++i) cptcls[b + i] = v[i];
   } else {
-    for (int i = b + 1; i < e; ++i) {
-      const int t = cptcls[i];
-      int j = i - 1;
-      while (j >= b && cptcls[j] > t) {
-        cptcls[j + 1] = cptcls[j];
-        --j;
-      }
-      cptcls[j + 1] = t;
-    }
+    heap_sort(cptcls + b, n);
   }
 }
 

@@ -1598,6 +1598,14 @@ int amsc_ens_status(amsc_ens *e, int *it, double *t, double *dt, int *active,
   return AMSC_OK;
 }
 
+int amsc_ens_diverged(amsc_ens *e, int *out) {
+  if (!e) return bad_handle();
+  if (!out) return fail(AMSC_ERR_INVALID, "NULL output");
+  if (int rc = read_ctrl(*e)) return rc;
+  for (int i = 0; i < e->S; ++i) out[i] = e->ctrl_host[i].diverged;
+  return AMSC_OK;
+}
+
 int amsc_ens_get_field(amsc_ens *e, int scen, const char *name, double *out,
                        int n) {
   if (!e) return bad_handle();

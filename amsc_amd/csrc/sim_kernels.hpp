@@ -20,6 +20,8 @@
 // tscen.
 #pragma once
 
+#include <cstddef>
+
 #include "binning.hpp"
 #include "common.hpp"
 #include "merge_split.hpp"
@@ -81,13 +83,21 @@ struct Ctrl {
   double cons[2];  // err_mass, err_mom of the last step
   int it;          // steps taken
   int active;      // 0 once the run has ended
-  int pad[2];
+  int diverged;    // 1: a particle left the physical range (run stopped)
+  int pad;
   // max |vpx|, |vpy|, hp of the particles as bit patterns of non-negative
   // doubles, by step parity, spread over 64 addresses so that the per-block
   // atomics of the particle kernel do not all hit one line
   unsigned long long cfl[2][64][3];
 };
-constexpr size_t kCtrlHead = 48;  // bytes before cfl (what the host reads)
+constexpr size_t kCtrlHead = 56;  // bytes before cfl (what the host reads)
+static_assert(offsetof(Ctrl, cfl) == kCtrlHead, "control block header size");
+
+// Divergence stop (no counterpart in optim_par, whose loop would step on
+// with a vanishing dt): a particle speed above this, or NaN, flags the
+// scenario and the next step's prologue ends its run.  Depth-averaged
+// flows move at tens of m/s.
+constexpr double kSpeedDiverged = 1.0e4;
 
 struct KParams {
   int nrows, ncols;
@@ -175,6 +185,10 @@ __global__ void __launch_bounds__(64)
   const KParams &P = scen[blockIdx.x].P;
   Ctrl *ctrl = &ctrls[blockIdx.x];
   if (!ctrl->active) return;
+  if (ctrl->diverged) {
+    if (lane == 0) ctrl->active = 0;
+    return;
+  }
   const bool fixed = P.dt_fixed > 0.0;
   const int it = ctrl->it;
   // loop condition, optim_par.cpp:281; optional step cap for adaptive runs
@@ -890,10 +904,13 @@ __global__ void __launch_bounds__(BS)
   int newkey = 0;
   double vmx = 0.0, vmy = 0.0, hmx = 0.0;
   if (valid) {
-    if (act)
+    if (act) {
       particle_body<R, AUX>(a, T.scen[sid], C.dt, ip, newkey, vmx, vmy, hmx);
-    else
+      if (!(vmx <= kSpeedDiverged && vmy <= kSpeedDiverged))
+        atomicOr(&T.ctrl[sid].diverged, 1);
+    } else {
       newkey = a.key_in[ip];
+    }
   }
   const int sl = wave_slot(a.count, newkey, valid);
   if (valid) {

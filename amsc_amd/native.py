@@ -158,6 +158,7 @@ def lib():
     L.amsc_ens_create.argtypes = [C.c_int, C.POINTER(AmscParams), _ip] + \
         [_dp] * 11 + [C.c_int, C.c_longlong, C.POINTER(_vp)]
     L.amsc_ens_status.argtypes = [_vp, _ip, _dp, _dp, _ip, _ip]
+    L.amsc_ens_diverged.argtypes = [_vp, _ip]
     L.amsc_ens_get_field.argtypes = [_vp, C.c_int, C.c_char_p, _dp, C.c_int]
     L.amsc_ens_get_field_int.argtypes = [_vp, C.c_int, C.c_char_p, _ip, C.c_int]
     L.amsc_ens_set_field.argtypes = [_vp, C.c_int, C.c_char_p, _dp, C.c_int]

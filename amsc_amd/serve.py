@@ -42,6 +42,7 @@ class Request:
     steps: int = 0
     result: dict | None = None
     cur_np: int = -1  # particle count while running (merge/split moves it)
+    diverged: bool = False  # run stopped by the divergence limit
 
     @property
     def np(self) -> int:
@@ -204,6 +205,7 @@ class BatchedServer:
             for i, r in enumerate(active):
                 r.steps = int(st["it"][i])
                 r.cur_np = int(st["np"][i])
+                r.diverged = bool(st["diverged"][i])
                 if st["active"][i]:
                     still.append(i)
                 elif math.isnan(r.done):
@@ -245,7 +247,8 @@ class BatchedServer:
         tts = [r.done - r.arrival for r in requests]
         fin = int(np.isfinite(tts).sum())
         return {"form": f"batched-{self.precision}", "requests": len(requests),
-                "finished": fin, "wall_s": wall, "scen_per_s": fin / wall,
+                "finished": fin, "diverged": int(sum(r.diverged for r in requests)),
+                "wall_s": wall, "scen_per_s": fin / wall,
                 "repacks": repacks, "repack_s": repack_s, **_pcts(tts)}
 
 
@@ -280,6 +283,7 @@ class SequentialServer:
                 now = clock() - t0
                 if k < self.chunk:
                     r.done = now
+                    r.diverged = s.diverged
                     break
                 if log and now - last_log >= progress_s:
                     last_log = now
@@ -296,4 +300,5 @@ class SequentialServer:
         tts = [r.done - r.arrival for r in requests]
         fin = int(np.isfinite(tts).sum())
         return {"form": "sequential-fp64", "requests": len(requests), "finished": fin,
+                "diverged": int(sum(r.diverged for r in requests)),
                 "wall_s": wall, "scen_per_s": fin / wall, **_pcts(tts)}

@@ -103,6 +103,14 @@ class Simulation:
                                                  C.byref(dt)))
         return it.value, t.value, dt.value
 
+    @property
+    def diverged(self) -> bool:
+        """True once the run was stopped because a particle speed exceeded
+        1e4 m/s or became NaN."""
+        out = np.zeros(1, np.int32)
+        native.check(native.lib().amsc_ens_diverged(self._h, out))
+        return bool(out[0])
+
     def get(self, name: str) -> np.ndarray:
         n = self.nn if name in NODAL_FIELDS else self.num_particles
         out = np.empty(max(n, 1), np.float64)
@@ -224,7 +232,10 @@ class Ensemble(Simulation):
         it, act, npart = (np.zeros(n, np.int32) for _ in range(3))
         t, dt = np.zeros(n), np.zeros(n)
         native.check(native.lib().amsc_ens_status(self._h, it, t, dt, act, npart))
-        return {"it": it, "t": t, "dt": dt, "active": act, "np": npart}
+        div = np.zeros(n, np.int32)
+        native.check(native.lib().amsc_ens_diverged(self._h, div))
+        return {"it": it, "t": t, "dt": dt, "active": act, "np": npart,
+                "diverged": div}
 
     def num(self, scen: int) -> int:
         return int(self.status()["np"][scen])

@@ -366,7 +366,8 @@ def served_bench(args):
                 f"{r['scen_per_s']:.2f} scen/s, p50 {r['p50_s']} s, p99 {r['p99_s']} s")
         # requests finished by both forms
         pairs = {f: [(q, p) for q, p in zip(done[f], done["fp64"])
-                     if q.result is not None and p.result is not None]
+                     if q.result is not None and p.result is not None
+                     and not q.diverged and not p.diverged]
                  for f in ("mixed", "fp64_batched")}
         res["mixed"]["error_vs_fp64"] = aggregate_errors(
             [field_errors(q.result, p.result, p.prm) for q, p in pairs["mixed"]])

@@ -340,6 +340,11 @@ int amsc_ens_create(int nscen, const amsc_params *params, const int *np,
  * running; *done is the largest number of steps any scenario took. */
 int amsc_ens_status(amsc_ens *e, int *it, double *t, double *dt, int *active,
                     int *np);
+/* 1 for every scenario whose run was stopped because a particle speed
+ * exceeded 1e4 m/s or became NaN (optim_par would step on with a vanishing
+ * dt); its state is that of the step that crossed the limit.  nscen
+ * entries; synchronises. */
+int amsc_ens_diverged(amsc_ens *e, int *out);
 int amsc_ens_get_field(amsc_ens *e, int scen, const char *name, double *out,
                        int n);
 int amsc_ens_get_field_int(amsc_ens *e, int scen, const char *name, int *out,

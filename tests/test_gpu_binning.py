@@ -57,6 +57,17 @@ def test_crowded_cells_and_sorted_input():
     check(xs, ys, nrows, ncols, hx, hy)
 
 
+def test_pile_up_in_one_cell():
+    """20000 particles in one cell (what clamping at the domain edge makes
+    of a run whose velocities explode): the in-cell sort stays O(n log n)."""
+    rng = np.random.default_rng(7)
+    nrows, ncols, hx, hy = 12, 10, 1.0, 1.0
+    x = np.concatenate([np.full(20000, 1e-10), rng.uniform(0, 10, 2000)])
+    y = np.concatenate([rng.uniform(0.0, 1.0, 20000), rng.uniform(0, 12, 2000)])
+    perm = rng.permutation(len(x))
+    check(x[perm], y[perm], nrows, ncols, hx, hy)
+
+
 def test_empty_and_clamped():
     check(np.zeros(0), np.zeros(0), 4, 5, 1.0, 1.0)
     # outside the domain: clamped to the border cells

@@ -283,6 +283,33 @@ def test_capacity_overflow_is_reported():
     e.close()
 
 
+def test_divergence_stops_only_the_diverging_scenario():
+    """A scenario whose particles move faster than the divergence limit
+    (1e4 m/s) is stopped after the step that crossed it and reported as
+    diverged; the scenario beside it runs on exactly as it would alone."""
+    from amsc_amd.simulation import Ensemble, Simulation
+    print("[ens] divergence stop", flush=True)
+    good = scen(small_slope_case())
+    prm, part, Z = scen(small_slope_case(seed=3))
+    part = dict(part)
+    part["vpx"] = np.full(len(part["x"]), 1.0e6)
+    part["mom_px"] = part["vpx"] * np.asarray(part["Mp"])
+    e = Ensemble([good, (prm, part, Z)], "fp64")
+    e.set_aux(True)
+    e.step(10)
+    st = e.status()
+    assert list(st["diverged"]) == [0, 1] and list(st["active"]) == [1, 0]
+    assert int(st["it"][0]) == 10 and int(st["it"][1]) == 1
+    o = oracle.OracleSim(*good)
+    o.run(10)
+    assert_scen_equal(e, 0, o, "beside a diverged scenario")
+    e.close()
+    s = Simulation(prm, part, Z)
+    s.step(5)
+    assert s.diverged and s.state()[0] == 1
+    s.close()
+
+
 def test_bytes_estimate_covers_allocation():
     from amsc_amd.simulation import Ensemble
     cases = mixed_cases()
