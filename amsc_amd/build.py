@@ -25,7 +25,7 @@ RUN = os.path.join(OUT_DIR, "amsc_run")
 
 HIP_SOURCES = ["runtime.hip", "dropin.hip", "binning.hip", "merge_split.hip",
                "sim.hip"]
-CXX_SOURCES = ["data_json.cpp"]
+CXX_SOURCES = ["data_json.cpp", "output.cpp"]
 ARCH = os.environ.get("AMSC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

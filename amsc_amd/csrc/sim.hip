@@ -1643,6 +1643,12 @@ int amsc_ens_ms_counts(amsc_ens *e, int scen, int out[2]) {
   return AMSC_OK;
 }
 
+int amsc_sim_params(amsc_sim *s, amsc_params *out) {
+  if (!s || !out) return fail(AMSC_ERR_INVALID, "bad arguments");
+  *out = s->prm[0];
+  return AMSC_OK;
+}
+
 int amsc_ens_info(amsc_ens *e, int *nscen, int *precision, long long *bytes,
                   int *capacity) {
   if (!e) return bad_handle();

@@ -139,6 +139,9 @@ def lib():
     L.amsc_sim_get_field_int.argtypes = [_vp, C.c_char_p, _ip, C.c_int]
     L.amsc_sim_set_field.argtypes = [_vp, C.c_char_p, _dp, C.c_int]
     L.amsc_sim_conservation.argtypes = [_vp, _dp]
+    L.amsc_sim_params.argtypes = [_vp, C.POINTER(AmscParams)]
+    L.amsc_sim_write_particles_csv.argtypes = [_vp, C.c_char_p]
+    L.amsc_sim_write_grid_vts.argtypes = [_vp, C.c_char_p]
     L.amsc_sim_ms_counts.argtypes = [_vp, _ip]
     L.amsc_sim_particle_steps.argtypes = [_vp, C.POINTER(C.c_longlong)]
     L.amsc_device_alloc.argtypes = [C.c_size_t, C.POINTER(_vp)]

@@ -135,6 +135,23 @@ class Simulation:
         native.check(native.lib().amsc_sim_conservation(self._h, out))
         return out
 
+    def params(self) -> dict:
+        """Solver parameters (scenario 0 of a batch) as a dict."""
+        p = native.AmscParams()
+        native.check(native.lib().amsc_sim_params(self._h, C.byref(p)))
+        return {name: getattr(p, name) for name, _ in p._fields_}
+
+    def write_particles_csv(self, path: str):
+        """nc_particles CSV of the current state (quoted header, columns by
+        name; output-only fields when the last step produced them)."""
+        native.check(native.lib().amsc_sim_write_particles_csv(self._h,
+                                                               str(path).encode()))
+
+    def write_grid_vts(self, path: str):
+        """VTK XML structured grid of Z and the nodal fields of the last step."""
+        native.check(native.lib().amsc_sim_write_grid_vts(self._h,
+                                                          str(path).encode()))
+
     @property
     def particle_steps(self) -> int:
         out = C.c_longlong()

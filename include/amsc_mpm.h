@@ -286,6 +286,22 @@ int amsc_sim_set_field(amsc_sim *s, const char *name, const double *in,
 /* err_mass, err_mom (optim_par.cpp:395-428, FP64 device sums) and t of the
  * last step. */
 int amsc_sim_conservation(amsc_sim *s, double out[3]);
+/* Parameters of the simulation (scenario 0 of a batch). */
+int amsc_sim_params(amsc_sim *s, amsc_params *out);
+/* The reference solver's output files (optim_par.cpp:269,311-337,655-665),
+ * written from the current state:
+ *   particles CSV (nc_particles_<it>.csv): a header of quoted field names,
+ *     "x","y", then the particle fields in alphabetical order, then "label"
+ *     and "level", one row per particle, values with 17 significant digits.
+ *     Output-only fields are written when the last step produced them
+ *     (amsc_sim_set_aux(s, 1)).  Quadgrid's own column order is not
+ *     available, so readers should select columns by name, as the
+ *     reference's scripts do;
+ *   grid VTS (GRID_forZ.vts, nc_grid_<it>.vts): a VTK XML StructuredGrid of
+ *     the (ncols+1) x (nrows+1) nodes, points (x, y, 0), point data Z and
+ *     the nodal fields the last step produced. */
+int amsc_sim_write_particles_csv(amsc_sim *s, const char *path);
+int amsc_sim_write_grid_vts(amsc_sim *s, const char *path);
 /* Particle-steps launched so far: the sum over steps of the particle count
  * at that step (steps issued after the end of a run are not counted once
  * the end has been observed). */
