@@ -20,6 +20,16 @@ data/dambreak/dambreak.m and data/performance/scaling.m:76-80).
   bench_slope(N)       the data/performance/scaling.m layout at N particles.
   ensemble_scenario()  one member of the hazard ensemble (seeded sizes and
                        rheology).
+  conservation_case()  data/conservation_test/conservation_test.m: dam break
+                       on an inclined plane with a Gaussian obstacle, 75x30
+                       cells over [0,150]x[0,20].
+  well_balance_case()  data/well_balance/well_balance.m: lake at rest
+                       (eq_level = 10) over a Gaussian bump, 50x50 cells.
+
+The last two restate the reference's Octave generators, including Octave's
+meshgrid/colon ordering, gradient() and interp2(..., 'linear') (the plane
+form a + b s + c t + d s t); Octave is not available here, so equality
+with the files Octave would write is not checked.
 """
 from __future__ import annotations
 
@@ -47,6 +57,45 @@ def _particles_in_cells(cells_c, cells_r, hx, hy, ppc_side=2):
     x = ((cells_c[:, None] + oc[None, :]) * hx).ravel()
     y = ((cells_r[:, None] + orr[None, :]) * hy).ravel()
     return x, y
+
+
+def _octave_range(a, inc, b):
+    """Octave's a:inc:b (elements a + k*inc)."""
+    n = int(math.floor((b - a) / inc * (1.0 + 4 * np.finfo(float).eps))) + 1
+    return a + np.arange(n) * inc
+
+
+def _octave_gradient(M, hx, hy):
+    """gradient(M, hx, hy) of Octave for a matrix indexed (row = y, column =
+    x): central differences inside, one-sided at the edges."""
+    def along(A, h):  # along axis 1
+        x = np.arange(A.shape[1]) * h
+        out = np.empty_like(A)
+        out[:, 0] = (A[:, 1] - A[:, 0]) / (x[1] - x[0])
+        out[:, -1] = (A[:, -1] - A[:, -2]) / (x[-1] - x[-2])
+        out[:, 1:-1] = (A[:, 2:] - A[:, :-2]) / (x[2:] - x[:-2])
+        return out
+    return along(M, hx), along(M.T, hy).T
+
+
+def _octave_interp2(xv, yv, M, xi, yi):
+    """interp2(X, Y, M, xi, yi, 'linear') of Octave on a meshgrid (xv along
+    columns, yv along rows): the containing cell (points on the last grid
+    line use the last cell), then a + b s + c t + d s t."""
+    j = np.clip(np.searchsorted(xv, xi, side="right") - 1, 0, len(xv) - 2)
+    i = np.clip(np.searchsorted(yv, yi, side="right") - 1, 0, len(yv) - 2)
+    s = (xi - xv[j]) / (xv[j + 1] - xv[j])
+    t = (yi - yv[i]) / (yv[i + 1] - yv[i])
+    a = M[i, j]
+    b = M[i, j + 1] - a
+    c = M[i + 1, j] - a
+    d = M[i + 1, j + 1] - a - b - c
+    return a + b * s + c * t + d * s * t
+
+
+def _colmajor(M):
+    """M(:) of Octave: column-major flattening (node id r + c*(rows))."""
+    return np.ascontiguousarray(M.T).ravel()
 
 
 def _base(nex, ney, hx, hy, x, y, hp, rho, Z, dZdx, dZdy, **kw):
@@ -282,3 +331,74 @@ def ensemble_scenario(seed, side_range=(64, 1024), np_range=(1e4, 1e6),
                  mu=float(rng.uniform(0.01, 1.0)), tauy=float(rng.uniform(1.0, 500.0)),
                  CFL=0.4, DT_FIXED=0.0, T=float(rng.uniform(20.0, 120.0)),
                  MERGE_SPLIT_ON=1, ms_call_interval=10)
+
+
+def conservation_case():
+    """data/conservation_test/conservation_test.m:8-76 and its DATA.json keys
+    (:107-146): domain [0,150]x[0,20], 75x30 cells, Z = 60 - x/3 +
+    15 exp(-((x-50)^2 + (y-10)^2)/7), particles on a 0.45 m lattice in
+    [0,20]^2 with hp = 60 - Z(xp) > 1 away from the boundary cells, equal
+    masses, Bingham + Voellmy, adaptive CFL 0.2 to T = 20, merge/split on."""
+    nex, ney, Lx, Ly = 75, 30, 150.0, 20.0
+    hx, hy = Lx / nex, Ly / ney
+    xv = np.linspace(0, Lx, nex + 1)
+    yv = np.linspace(0, Ly, ney + 1)
+    X, Y = np.meshgrid(xv, yv)
+    Z = 60 - (1 / 3) * X + 15 * np.exp(-((X - 50) ** 2) / 7 - ((Y - 10) ** 2) / 7)
+    gx, gy = _octave_gradient(Z, hx, hy)
+    DX = DY = 0.45
+    xp, yp = np.meshgrid(_octave_range(DX / 2, DX, 20 - DX / 2),
+                         _octave_range(DY / 2, DY, 20 - DY / 2))
+    xp, yp = _colmajor(xp), _colmajor(yp)
+    hp = 60 - _octave_interp2(xv, yv, Z, xp, yp)
+    keep = hp > 1.0
+    xp, yp, hp = xp[keep], yp[keep], hp[keep]
+    edge = (xp < hx) | (xp > Lx - hx) | (yp < hy) | (yp > Ly - hy)
+    xp, yp, hp = xp[~edge], yp[~edge], hp[~edge]
+    n = len(xp)
+    rho = 1200.0
+    Msys = np.sum(hp * DX * DY * rho)
+    Mp = Msys / n * np.ones(n)
+    Vp = Mp / rho
+    Ap = Vp / hp
+    d = _base(nex, ney, hx, hy, xp, yp, hp, rho, _colmajor(Z), _colmajor(gx),
+              _colmajor(gy), xi=200.0, T=20.0, mu=10.0, phi=12.0, tauy=30.0,
+              MERGE_SPLIT_ON=1, BINGHAM_ON=1.0, CFL=0.2, BC_FLAG=1,
+              eq_level=0.0, DT_FIXED=0.0, NSTEPS=100, FRICTION_ON=1.0,
+              ms_alpha=1.5, ms_split_hp_min=0.3, ms_beta=0.3)
+    d.update(Mp=Mp, Vp=Vp, Ap=Ap)
+    return d
+
+
+def well_balance_case():
+    """data/well_balance/well_balance.m:6-125 and its DATA.json keys
+    (:186-226): 10x10 m, 50x50 cells, Z = 5 exp(-0.4((x-5)^2 + (y-5)^2)),
+    eq_level = 10, particles on a 0.17 m lattice over [0,10]^2 with
+    hp = interp2(10 - Z), Ap = 0.17^2, Vp = Ap hp, rho = 1200, no rheology,
+    no boundary conditions, adaptive CFL 0.1 to T = 1, merge/split on."""
+    Lx = Ly = 10.0
+    nex = ney = 50
+    hx, hy = Lx / nex, Ly / ney
+    xv = _octave_range(0.0, hx, Lx)
+    yv = _octave_range(0.0, hy, Ly)
+    X, Y = np.meshgrid(xv, yv)
+    Z = 5.0 * np.exp(-2 / 5 * ((X - 5) ** 2 + (Y - 5) ** 2))
+    gx, gy = _octave_gradient(Z, hx, hy)
+    h = 10 - Z
+    h[h < 0] = 0
+    DX = DY = 0.17
+    xp, yp = np.meshgrid(_octave_range(0.0, DX, 10.0), _octave_range(0.0, DY, 10.0))
+    hp = _colmajor(_octave_interp2(xv, yv, h, xp, yp))
+    xp, yp = _colmajor(xp), _colmajor(yp)
+    n = len(xp)
+    rho = 1200.0
+    Ap = DX * DY * np.ones(n)
+    Vp = Ap * hp
+    Mp = rho * Vp
+    d = _base(nex, ney, hx, hy, xp, yp, hp, rho, _colmajor(Z), _colmajor(gx),
+              _colmajor(gy), xi=200.0, T=1.0, mu=2000.0, phi=20.0, tauy=30.0,
+              BC_FLAG=0, CFL=0.1, BINGHAM_ON=0.0, FRICTION_ON=0.0, DT_FIXED=0.0,
+              NSTEPS=100, MERGE_SPLIT_ON=1, ms_alpha=1.3, ms_beta=0.3,
+              eq_level=10.0)
+    d.update(Mp=Mp, Vp=Vp, Ap=Ap)
+    return d

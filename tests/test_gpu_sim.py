@@ -97,6 +97,26 @@ def test_reference_fixture_bitwise():
     assert_same(o, g, "step 100")
 
 
+@pytest.mark.parametrize("case,steps", [("conservation", 80), ("well_balance", 60),
+                                        ("fena", 40)])
+def test_reference_cases_bitwise(case, steps):
+    """The reference's own cases: data/conservation_test/conservation_test.m,
+    data/well_balance/well_balance.m (restated generators) and the real
+    terrain of data/fena/DATA.json (committed fixture)."""
+    print(f"[sim] reference case {case}", flush=True)
+    if case == "conservation":
+        d = scenarios.conservation_case()
+    elif case == "well_balance":
+        d = scenarios.well_balance_case()
+    else:
+        d = load_data_json(os.path.join(GOLDEN, "DATA_fena.json"))
+    o, g, _ = pair(d, capacity=4 * len(d["x"]))
+    advance(o, g, 1)
+    assert_same(o, g, f"{case} step 1")
+    advance(o, g, steps - 1)
+    assert_same(o, g, f"{case} step {steps}")
+
+
 def test_json_loader_matches_python_loader():
     from amsc_amd.simulation import Simulation
     from amsc_amd import native
