@@ -25,8 +25,10 @@ data/dambreak/dambreak.m and data/performance/scaling.m:76-80).
                        cells over [0,150]x[0,20].
   well_balance_case()  data/well_balance/well_balance.m: lake at rest
                        (eq_level = 10) over a Gaussian bump, 50x50 cells.
+  dambreak_case()      data/dambreak/dambreak.m: a 5 m column released down
+                       a 200x60-cell channel sloping at 5 %.
 
-The last two restate the reference's Octave generators, including Octave's
+The last three restate the reference's Octave generators, including Octave's
 meshgrid/colon ordering, gradient() and interp2(..., 'linear') (the plane
 form a + b s + c t + d s t); Octave is not available here, so equality
 with the files Octave would write is not checked.
@@ -400,5 +402,40 @@ def well_balance_case():
               BC_FLAG=0, CFL=0.1, BINGHAM_ON=0.0, FRICTION_ON=0.0, DT_FIXED=0.0,
               NSTEPS=100, MERGE_SPLIT_ON=1, ms_alpha=1.3, ms_beta=0.3,
               eq_level=10.0)
+    d.update(Mp=Mp, Vp=Vp, Ap=Ap)
+    return d
+
+
+def dambreak_case():
+    """data/dambreak/dambreak.m:7-113 and its DATA.json keys (:171-204):
+    [0,100]x[0,30] m, 200x60 cells, Z = max(5 - 0.05 x, 0), particles on a
+    0.5 m lattice with hp = 5 where 1 <= x <= 50, equal masses, rho = 1000,
+    no rheology, adaptive CFL 0.2 to T = 8, merge/split on."""
+    nex, ney = 200, 60
+    xv = np.linspace(0, 100, nex + 1)
+    yv = np.linspace(0, 30, ney + 1)
+    hx, hy = 100 / nex, 30 / ney
+    X, Y = np.meshgrid(xv, yv)
+    Z = 5 - 0.05 * X
+    Z[Z < 0] = 0
+    DX = DY = 0.5
+    xp, yp = np.meshgrid(_octave_range(0.0, DX, 100.0), _octave_range(0.0, DY, 30.0))
+    hp = 5.0 * np.ones_like(xp)
+    hp[(xp < 1) | (xp > 50)] = 0
+    hp, xp, yp = _colmajor(hp), _colmajor(xp), _colmajor(yp)
+    keep = hp > 0.001
+    hp, xp, yp = hp[keep], xp[keep], yp[keep]
+    gx, gy = _octave_gradient(Z, hx, hy)
+    n = len(xp)
+    rho = 1000.0
+    Msys = np.sum(hp * DX * DY * rho)
+    Mp = Msys / n * np.ones(n)
+    Vp = Mp / rho
+    Ap = Vp / hp
+    d = _base(nex, ney, hx, hy, xp, yp, hp, rho, _colmajor(Z), _colmajor(gx),
+              _colmajor(gy), xi=200.0, T=8.0, mu=50.0, phi=37.0, tauy=2000.0,
+              MERGE_SPLIT_ON=1, ms_alpha=1.3, ms_beta=0.3, BINGHAM_ON=0.0,
+              FRICTION_ON=0.0, CFL=0.2, eq_level=0.0, BC_FLAG=1,
+              DT_FIXED=0.0, NSTEPS=0)
     d.update(Mp=Mp, Vp=Vp, Ap=Ap)
     return d

@@ -98,16 +98,19 @@ def test_reference_fixture_bitwise():
 
 
 @pytest.mark.parametrize("case,steps", [("conservation", 80), ("well_balance", 60),
-                                        ("fena", 40)])
+                                        ("fena", 40), ("dambreak", 60)])
 def test_reference_cases_bitwise(case, steps):
     """The reference's own cases: data/conservation_test/conservation_test.m,
-    data/well_balance/well_balance.m (restated generators) and the real
-    terrain of data/fena/DATA.json (committed fixture)."""
+    data/well_balance/well_balance.m, data/dambreak/dambreak.m (restated
+    generators) and the real terrain of data/fena/DATA.json (committed
+    fixture)."""
     print(f"[sim] reference case {case}", flush=True)
     if case == "conservation":
         d = scenarios.conservation_case()
     elif case == "well_balance":
         d = scenarios.well_balance_case()
+    elif case == "dambreak":
+        d = scenarios.dambreak_case()
     else:
         d = load_data_json(os.path.join(GOLDEN, "DATA_fena.json"))
     o, g, _ = pair(d, capacity=4 * len(d["x"]))

@@ -8,7 +8,8 @@ reference's scripts:
   plane with an obstacle): err_mass, err_mom below 1e-10 at every step
   (data/conservation_test/scripts/conservation_check.py:36);
 * data/fena/DATA.json (a real terrain, committed as tests/golden/
-  DATA_fena.json): the same conservation bound over 30 steps.
+  DATA_fena.json) and data/dambreak/dambreak.m (a column released down a
+  sloping channel): the same conservation bound.
 """
 import os
 
@@ -57,3 +58,10 @@ def test_fena_terrain_transfers_to_round_off():
     assert len(d["x"]) == 16430 and (d["Nex"], d["Ney"]) == (259, 350)
     o, done, worst = _run(d, 30)
     assert done == 30 and worst < 1e-10
+
+
+def test_dambreak_case_transfers_to_round_off():
+    d = scenarios.dambreak_case()
+    assert len(d["x"]) == 99 * 61  # x in [1, 50], y in [0, 30] at 0.5 m
+    o, done, worst = _run(d, 100)
+    assert done == 100 and worst < 1e-10
