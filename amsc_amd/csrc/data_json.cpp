@@ -272,6 +272,8 @@ extern "C" int amsc_sim_create_from_json(const char *path, int max_steps,
   p.ms_min_ppc = (int)opt("ms_min_particles_per_cell", 2);
   p.ms_interval = (int)opt("ms_call_interval", 10);
   p.ms_max_ops = opt("ms_max_ops", 1e9);
+  p.ms_mode = (int)opt("ms_mode", 0);
+  p.ms_stretch = opt("ms_stretch_thresh", 1e7);
   const size_t np = pa[0]->size();
   for (int i = 1; i < 10; ++i)
     if (pa[i]->size() != np)

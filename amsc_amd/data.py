@@ -20,6 +20,10 @@ MS_DEFAULTS = {
     "ms_shear_split": 1e30, "ms_hp_min": 0.05, "ms_max_dv": 0.01,
     "ms_min_level": -2, "ms_max_level": 2, "ms_min_particles_per_cell": 2,
     "ms_call_interval": 10, "ms_max_ops": 1e9,
+    # this build's keys: merge/split rule (0: merge_split_ops_cmes.h, 1: the
+    # legacy ELFS rule of merge_split_ops.h) and the ELFS divergence trigger
+    # (ms_config::stretch_thresh, merge_split_ops.h:27)
+    "ms_mode": 0, "ms_stretch_thresh": 1e7,
 }
 
 
@@ -85,6 +89,7 @@ def params_from_data(d: dict, max_steps: int = 0) -> dict:
         "ms_max_level": int(d["ms_max_level"]),
         "ms_min_ppc": int(d["ms_min_particles_per_cell"]),
         "ms_interval": int(d["ms_call_interval"]),
+        "ms_mode": int(d["ms_mode"]), "ms_stretch": float(d["ms_stretch_thresh"]),
     }
 
 

@@ -42,6 +42,7 @@ class AmscParams(C.Structure):
         ("ms_max_ops", C.c_double),
         ("ms_min_level", C.c_int), ("ms_max_level", C.c_int),
         ("ms_min_ppc", C.c_int), ("ms_interval", C.c_int),
+        ("ms_mode", C.c_int), ("ms_stretch", C.c_double),
     ]
 
     @classmethod

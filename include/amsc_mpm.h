@@ -219,6 +219,12 @@ typedef struct amsc_params {
   double ms_alpha, ms_beta, ms_split_hp_min, ms_shear_split, ms_hp_min,
       ms_max_dv, ms_max_ops;
   int ms_min_level, ms_max_level, ms_min_ppc, ms_interval;
+  /* merge/split rule: 0 = merge_split_ops_cmes.h (the solvers' rule),
+   * 1 = the legacy ELFS rule of merge_split_ops.h (split where the distance
+   * to the fluid boundary is below ms_alpha * sqrt(Ap) or the divergence
+   * exceeds ms_stretch, merge where it is above ms_beta * sqrt(Ap)) */
+  int ms_mode;
+  double ms_stretch;
 } amsc_params;
 
 /* decide_actions (merge_split_ops_cmes.h:116-253) on the device, for a state

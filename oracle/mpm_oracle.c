@@ -794,18 +794,16 @@ static void principal_stretch_dir(double exx, double eyy, double exy,
 
 /* decide_actions, merge_split_ops_cmes.h:116-253, with the cells of the
  * given CSR (ascending cell = grd_to_ptcl's map order). */
-static void ms_decide_csr(const or_params *p, int np, int nc, double dt,
+static void ms_flags_cmes(const or_params *p, int np, int nc, double dt,
                           const double *x, const double *y, const double *Ap,
                           const double *hp, const double *exx_,
                           const double *exy1, const double *exy2,
                           const double *eyy_, const double *vpx,
-                          const double *vpy, const int *lvl,
-                          const int *cstart, const int *cptcls, int *act,
-                          int *partner, int counts[2]) {
+                          const double *vpy, const int *lvl, int *act,
+                          int *partner) {
   const double min_h = p->hx < p->hy ? p->hx : p->hy;
   const double r_split = p->ms_alpha * min_h;
   const double r_merge = p->ms_beta * min_h;
-  const int max_ops = (int)p->ms_max_ops; /* gpu_offload.cpp:317 */
   for (int ip = 0; ip < np; ++ip) {
     act[ip] = 0;
     partner[ip] = -1;
@@ -822,6 +820,23 @@ static void ms_decide_csr(const or_params *p, int np, int nc, double dt,
     else if (thick && hp[ip] > p->ms_hp_min && lvl[ip] < p->ms_max_level)
       act[ip] = 2;
   }
+}
+
+/* The per-cell part of decide_actions (merge_split_ops_cmes.h:176-249 and,
+ * with split_w = 4 and min_ppc = 2, merge_split_ops.h:405-497): over the
+ * cells in ascending order, each cell's particles in ascending id order
+ * (the quadgrid order assumed throughout): the global split cap, greedy
+ * nearest-partner pairing of the merge candidates under the global merge
+ * cap, candidates left unpaired kept, then merges undone (first primary in
+ * id order) until the cell keeps min_ppc particles.  act holds the flags on
+ * entry (1 split, 2 merge candidate) and the actions on exit. */
+static void ms_pair_csr(const or_params *p, int np, int nc, const double *x,
+                        const double *y, const double *vpx, const double *vpy,
+                        const int *cstart, const int *cptcls, int *act,
+                        int *partner, int counts[2], int split_w,
+                        int min_ppc) {
+  const int max_ops = (int)p->ms_max_ops; /* gpu_offload.cpp:317 */
+  for (int ip = 0; ip < np; ++ip) partner[ip] = -1;
   const double max_dv2 = p->ms_max_dv * p->ms_max_dv;
   int n_split = 0, n_merge = 0;
   int *cand = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
@@ -872,9 +887,9 @@ static void ms_decide_csr(const or_params *p, int np, int nc, double dt,
     int projected = 0;
     for (int j = b; j < e; ++j) {
       const int a = act[cptcls[j]];
-      projected += a == 0 ? 1 : a == 1 ? 2 : a == 2 ? 1 : 0;
+      projected += a == 0 ? 1 : a == 1 ? split_w : a == 2 ? 1 : 0;
     }
-    while (projected < p->ms_min_ppc) {
+    while (projected < min_ppc) {
       int undone = 0;
       for (int j = b; j < e; ++j) {
         const int q = cptcls[j];
@@ -912,8 +927,10 @@ void or_ms_decide(const or_params *p, int np, double dt, const double *x,
   int coloff[5];
   or_bin_particles(np, x, y, p->nrows, p->ncols, p->hx, p->hy, p2g, cstart,
                    cptcls, ccidx, coloff);
-  ms_decide_csr(p, np, nc, dt, x, y, Ap, hp, vpx_dx, vpx_dy, vpy_dx, vpy_dy,
-                vpx, vpy, level, cstart, cptcls, action, partner, counts);
+  ms_flags_cmes(p, np, nc, dt, x, y, Ap, hp, vpx_dx, vpx_dy, vpy_dx, vpy_dy,
+                vpx, vpy, level, action, partner);
+  ms_pair_csr(p, np, nc, x, y, vpx, vpy, cstart, cptcls, action, partner,
+              counts, 2, p->ms_min_ppc);
   free(p2g); free(cptcls); free(cstart); free(ccidx);
 }
 
@@ -996,6 +1013,235 @@ static void ms_execute(or_sim *s, const int *act, const int *partner) {
   }
 #undef CLAMPX
 #undef CLAMPY
+  sim_reserve(s, Nnew > 0 ? Nnew : 1);
+  for (int k = 0; k < OR_NFIELDS; ++k) {
+    memcpy(s->f[k], nf[k], sizeof(double) * Nnew);
+    free(nf[k]);
+  }
+  memcpy(s->label, nlabel, sizeof(int) * Nnew);
+  memcpy(s->level, nlevel, sizeof(int) * Nnew);
+  free(nlabel); free(nlevel); free(offset);
+  s->np = Nnew;
+}
+
+/* ---- legacy ELFS merge/split (merge_split_ops.h:86-755), ms_mode 1 ---- */
+
+/* mark_exterior_cells, merge_split_ops.h:86-165: the empty cells reachable
+ * from the grid edge's empty cells through empty cells (4-neighbours). */
+static void elfs_exterior(int nrows, int ncols, const int *cstart, char *ext,
+                          int *queue) {
+  const int nc = nrows * ncols;
+  int head = 0, tail = 0;
+  for (int i = 0; i < nc; ++i) ext[i] = 0;
+#define WET(i) (cstart[(i) + 1] > cstart[i])
+  for (int c = 0; c < ncols; ++c) {
+    const int rs[2] = {0, nrows - 1};
+    for (int k = 0; k < 2; ++k) {
+      const int idx = rs[k] + nrows * c;
+      if (!WET(idx) && !ext[idx]) { ext[idx] = 1; queue[tail++] = idx; }
+    }
+  }
+  for (int r = 0; r < nrows; ++r) {
+    const int cs[2] = {0, ncols - 1};
+    for (int k = 0; k < 2; ++k) {
+      const int idx = r + nrows * cs[k];
+      if (!WET(idx) && !ext[idx]) { ext[idx] = 1; queue[tail++] = idx; }
+    }
+  }
+  static const int dr[4] = {-1, 1, 0, 0}, dc[4] = {0, 0, -1, 1};
+  while (head < tail) {
+    const int cidx = queue[head++];
+    const int r = cidx % nrows, c = cidx / nrows;
+    for (int k = 0; k < 4; ++k) {
+      const int nr = r + dr[k], ncl = c + dc[k];
+      if (nr < 0 || nr >= nrows || ncl < 0 || ncl >= ncols) continue;
+      const int n = nr + nrows * ncl;
+      if (!WET(n) && !ext[n]) { ext[n] = 1; queue[tail++] = n; }
+    }
+  }
+}
+
+/* compute_boundary_distance, merge_split_ops.h:170-262: hop count (times
+ * min(hx, hy)) from the wet cells off the domain edge with an exterior
+ * cell to their left or right, through wet cells; DBL_MAX elsewhere. */
+static void elfs_distance(int nrows, int ncols, double h, const int *cstart,
+                          const char *ext, double *dist, int *qc, int *qd) {
+  const int nc = nrows * ncols;
+  int head = 0, tail = 0;
+  for (int i = 0; i < nc; ++i) dist[i] = 1.7976931348623157e308; /* DBL_MAX */
+  for (int c = 0; c < ncols; ++c)
+    for (int r = 0; r < nrows; ++r) {
+      const int idx = r + nrows * c;
+      if (!WET(idx)) continue;
+      const int edge = r == 0 || r == nrows - 1 || c == 0 || c == ncols - 1;
+      int on = 0;
+      if (!edge) {
+        if (c > 0 && ext[idx - nrows]) on = 1;
+        if (c < ncols - 1 && ext[idx + nrows]) on = 1;
+      }
+      if (on) { dist[idx] = 0.0; qc[tail] = idx; qd[tail] = 0; ++tail; }
+    }
+  static const int dr[4] = {-1, 1, 0, 0}, dc[4] = {0, 0, -1, 1};
+  while (head < tail) {
+    const int cidx = qc[head], d = qd[head];
+    ++head;
+    const int r = cidx % nrows, c = cidx / nrows;
+    for (int k = 0; k < 4; ++k) {
+      const int nr = r + dr[k], ncl = c + dc[k];
+      if (nr < 0 || nr >= nrows || ncl < 0 || ncl >= ncols) continue;
+      const int n = nr + nrows * ncl;
+      if (!WET(n)) continue;
+      const double nd = (d + 1) * h;
+      if (nd < dist[n]) { dist[n] = nd; qc[tail] = n; qd[tail] = d + 1; ++tail; }
+    }
+  }
+#undef WET
+}
+
+/* compute_elfs, merge_split_ops.h:266-342: the cell distance, refined near
+ * the boundary by the distance from the particle to the exterior cells of
+ * its row within two columns. */
+static double elfs_one(const or_params *p, int cell, double px, double py,
+                       const double *dist, const char *ext) {
+  const int nrows = p->nrows, ncols = p->ncols;
+  const double hx = p->hx, hy = p->hy;
+  const double thr = hx > hy ? hx : hy;
+  const double base = dist[cell];
+  if (base > thr) return base;
+  double mind = base;
+  const int r0 = cell % nrows, c0 = cell / nrows;
+  for (int dcl = -2; dcl <= 2; ++dcl) {
+    const int r = r0, c = c0 + dcl;
+    if (r < 0 || r >= nrows || c < 0 || c >= ncols) continue;
+    const int n = r + nrows * c;
+    if (!ext[n]) continue;
+    const double cx_lo = c * hx, cx_hi = (c + 1) * hx;
+    const double cy_lo = r * hy, cy_hi = (r + 1) * hy;
+    const double nx = px < cx_lo ? cx_lo : (px > cx_hi ? cx_hi : px);
+    const double ny = py < cy_lo ? cy_lo : (py > cy_hi ? cy_hi : py);
+    const double dx = px - nx, dy = py - ny;
+    const double d = sqrt(dx * dx + dy * dy);
+    if (d < mind) mind = d;
+  }
+  return mind;
+}
+
+/* The two cell passes for a given occupancy (cstart), exported for tests. */
+void or_elfs_cells(int nrows, int ncols, double h, const int *cstart,
+                   char *ext, double *dist) {
+  const int nc = nrows * ncols;
+  int *q1 = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
+  int *q2 = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
+  elfs_exterior(nrows, ncols, cstart, ext, q1);
+  elfs_distance(nrows, ncols, h, cstart, ext, dist, q1, q2);
+  free(q1); free(q2);
+}
+
+/* decide_actions, merge_split_ops.h:349-510 (flags :388-402, then the
+ * per-cell pass with a 1 -> 4 split and at least 2 particles per cell). */
+static void elfs_decide(or_sim *s, int *act, int *partner, int counts[2]) {
+  const or_params *p = &s->p;
+  const int np = s->np, nc = s->nc;
+  char *ext = (char *)malloc(nc > 0 ? nc : 1);
+  double *dist = (double *)malloc(sizeof(double) * (nc > 0 ? nc : 1));
+  int *q1 = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
+  int *q2 = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
+  elfs_exterior(p->nrows, p->ncols, s->cstart, ext, q1);
+  elfs_distance(p->nrows, p->ncols, p->hx < p->hy ? p->hx : p->hy, s->cstart,
+                ext, dist, q1, q2);
+  double **f = s->f;
+  for (int ip = 0; ip < np; ++ip) {
+    const double e = elfs_one(p, s->p2g[ip], f[OR_X][ip], f[OR_Y][ip], dist, ext);
+    const double r_i = sqrt(f[OR_AP][ip]);
+    const double divergence = f[OR_VPX_DX][ip] + f[OR_VPY_DY][ip];
+    act[ip] = 0;
+    if ((e < p->ms_alpha * r_i) || (divergence > p->ms_stretch)) {
+      if (s->level[ip] > p->ms_min_level) act[ip] = 1;
+    } else if (e > p->ms_beta * r_i && f[OR_HP][ip] > p->ms_hp_min) {
+      if (s->level[ip] < p->ms_max_level) act[ip] = 2;
+    }
+  }
+  ms_pair_csr(p, np, nc, f[OR_X], f[OR_Y], f[OR_VPX], f[OR_VPY], s->cstart,
+              s->cptcls, act, partner, counts, 4, 2);
+  free(ext); free(dist); free(q1); free(q2);
+}
+
+/* execute_merge_split, merge_split_ops.h:513-716: weights {1,4,1,0}; a
+ * split leaves four children at (x +- r/2, y +- r/2), r = sqrt(Ap), clamped
+ * to the domain, extensive fields quartered, level - 1, label -1; a merge
+ * averages the intensive fields by mass, sums the extensive ones, keeps
+ * the primary's label, level + 1; momentum = Mp * v for both. */
+static void elfs_execute(or_sim *s, const int *act, const int *partner) {
+  const int N = s->np;
+  static const int weights[4] = {1, 4, 1, 0};
+  int *offset = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+  int Nnew = 0;
+  for (int i = 0; i < N; ++i) {
+    offset[i] = Nnew;
+    Nnew += weights[act[i]];
+  }
+  double *nf[OR_NFIELDS];
+  for (int k = 0; k < OR_NFIELDS; ++k)
+    nf[k] = (double *)malloc(sizeof(double) * (Nnew > 0 ? Nnew : 1));
+  int *nlabel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
+  int *nlevel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
+  const double Lx = s->p.ncols * s->p.hx, Ly = s->p.nrows * s->p.hy;
+  double **f = s->f;
+  for (int i = 0; i < N; ++i) {
+    const int a = act[i];
+    if (a == 0) {
+      const int o = offset[i];
+      for (int k = 0; k < OR_NFIELDS; ++k) nf[k][o] = f[k][i];
+      nlabel[o] = s->label[i];
+      nlevel[o] = s->level[i];
+    } else if (a == 1) {
+      const int o = offset[i];
+      const double px = f[OR_X][i], py = f[OR_Y][i];
+      const double off = sqrt(f[OR_AP][i]) / 2.0;
+      double cx[4] = {px + off, px + off, px - off, px - off};
+      double cy[4] = {py + off, py - off, py + off, py - off};
+      for (int k = 0; k < 4; ++k) {
+        if (cx[k] < 1e-10 || cx[k] > Lx - 1e-10) {
+          const double m = Lx - 1e-10 < cx[k] ? Lx - 1e-10 : cx[k];
+          cx[k] = 1e-10 > m ? 1e-10 : m;
+        }
+        if (cy[k] < 1e-10 || cy[k] > Ly - 1e-10) {
+          const double m = Ly - 1e-10 < cy[k] ? Ly - 1e-10 : cy[k];
+          cy[k] = 1e-10 > m ? 1e-10 : m;
+        }
+      }
+      for (int c = 0; c < 4; ++c) {
+        for (int k = 0; k < OR_NFIELDS; ++k) {
+          if (k == OR_X || k == OR_Y) continue;
+          nf[k][o + c] = f[k][i] / (is_extensive(k) ? 4.0 : 1.0);
+        }
+        nf[OR_X][o + c] = cx[c];
+        nf[OR_Y][o + c] = cy[c];
+        nf[OR_XP][o + c] = cx[c];
+        nf[OR_YP][o + c] = cy[c];
+        nf[OR_MOM_PX][o + c] = nf[OR_MP][o + c] * nf[OR_VPX][o + c];
+        nf[OR_MOM_PY][o + c] = nf[OR_MP][o + c] * nf[OR_VPY][o + c];
+        nlevel[o + c] = s->level[i] - 1;
+        nlabel[o + c] = -1;
+      }
+    } else if (a == 2) {
+      const int j = partner[i], o = offset[i];
+      const double M1 = f[OR_MP][i], M2 = f[OR_MP][j], Mt = M1 + M2;
+      const double nx = (M1 * f[OR_X][i] + M2 * f[OR_X][j]) / Mt;
+      const double ny = (M1 * f[OR_Y][i] + M2 * f[OR_Y][j]) / Mt;
+      for (int k = 0; k < OR_NFIELDS; ++k) {
+        if (k == OR_X || k == OR_Y) continue;
+        nf[k][o] = is_extensive(k) ? (f[k][i] + f[k][j])
+                                   : (M1 * f[k][i] + M2 * f[k][j]) / Mt;
+      }
+      nf[OR_X][o] = nx; nf[OR_Y][o] = ny;
+      nf[OR_XP][o] = nx; nf[OR_YP][o] = ny;
+      nf[OR_MOM_PX][o] = nf[OR_MP][o] * nf[OR_VPX][o];
+      nf[OR_MOM_PY][o] = nf[OR_MP][o] * nf[OR_VPY][o];
+      nlabel[o] = s->label[i];
+      nlevel[o] = s->level[i] + 1;
+    }
+  }
   sim_reserve(s, Nnew > 0 ? Nnew : 1);
   for (int k = 0; k < OR_NFIELDS; ++k) {
     memcpy(s->f[k], nf[k], sizeof(double) * Nnew);
@@ -1097,16 +1343,23 @@ int or_sim_step(or_sim *s) {
     const int N = s->np;
     int *act = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
     int *partner = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
-    ms_decide_csr(p, N, s->nc, dt, s->f[OR_X], s->f[OR_Y], s->f[OR_AP],
-                  s->f[OR_HP], s->f[OR_VPX_DX], s->f[OR_VPX_DY],
-                  s->f[OR_VPY_DX], s->f[OR_VPY_DY], s->f[OR_VPX],
-                  s->f[OR_VPY], s->level, s->cstart, s->cptcls, act, partner,
-                  s->ms_counts);
+    if (p->ms_mode == 1) {
+      elfs_decide(s, act, partner, s->ms_counts);
+    } else {
+      ms_flags_cmes(p, N, s->nc, dt, s->f[OR_X], s->f[OR_Y], s->f[OR_AP],
+                    s->f[OR_HP], s->f[OR_VPX_DX], s->f[OR_VPX_DY],
+                    s->f[OR_VPY_DX], s->f[OR_VPY_DY], s->f[OR_VPX],
+                    s->f[OR_VPY], s->level, act, partner);
+      ms_pair_csr(p, N, s->nc, s->f[OR_X], s->f[OR_Y], s->f[OR_VPX],
+                  s->f[OR_VPY], s->cstart, s->cptcls, act, partner,
+                  s->ms_counts, 2, p->ms_min_ppc);
+    }
     int any = 0;
     for (int i = 0; i < N; ++i)
       if (act[i] != 0) { any = 1; break; }
     if (any) {
-      ms_execute(s, act, partner);
+      if (p->ms_mode == 1) elfs_execute(s, act, partner);
+      else ms_execute(s, act, partner);
       sim_bin(s);
     }
     free(act);

@@ -143,6 +143,8 @@ typedef struct or_params {
   double ms_alpha, ms_beta, ms_split_hp_min, ms_shear_split, ms_hp_min,
       ms_max_dv, ms_max_ops;
   int ms_min_level, ms_max_level, ms_min_ppc, ms_interval;
+  int ms_mode;       /* 0: merge_split_ops_cmes.h, 1: merge_split_ops.h ELFS */
+  double ms_stretch; /* ELFS divergence trigger (ms_config::stretch_thresh) */
 } or_params;
 
 /* stress_tensor_t (optim_par.cpp:68-120) over np particles. */
@@ -162,6 +164,11 @@ enum {
 };
 
 typedef struct or_sim or_sim;
+
+/* Exterior cells and boundary distance of the legacy ELFS merge/split
+ * (merge_split_ops.h:86-262) for the occupancy given by cstart. */
+void or_elfs_cells(int nrows, int ncols, double h, const int *cstart,
+                   char *ext, double *dist);
 
 /* Threads of the per-particle loops (OpenMP builds; no effect otherwise). */
 void or_set_threads(int n);

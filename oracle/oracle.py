@@ -42,6 +42,7 @@ class OrParams(C.Structure):
         ("ms_max_ops", C.c_double),
         ("ms_min_level", C.c_int), ("ms_max_level", C.c_int),
         ("ms_min_ppc", C.c_int), ("ms_interval", C.c_int),
+        ("ms_mode", C.c_int), ("ms_stretch", C.c_double),
     ]
 
 
@@ -122,6 +123,25 @@ def bin_particles(x, y, nrows, ncols, hx, hy):
     lib().or_bin_particles(np_, x, y, nrows, ncols, hx, hy, p2g, cstart,
                            cptcls, ccidx, coloff)
     return p2g[:np_], cstart, cptcls[:np_], ccidx[:nc], coloff
+
+
+def elfs_cells(counts, h):
+    """Exterior flags and boundary distance of the legacy ELFS merge/split
+    for per-cell particle counts given as an (nrows, ncols) array indexed
+    [r, c] (cell id r + c*nrows)."""
+    counts = np.asarray(counts)
+    nrows, ncols = counts.shape
+    cstart = np.zeros(nrows * ncols + 1, np.int32)
+    cstart[1:] = np.cumsum(counts.T.ravel())
+    ext = np.zeros(max(nrows * ncols, 1), np.int8)
+    dist = np.zeros(max(nrows * ncols, 1), np.float64)
+    L = lib()
+    L.or_elfs_cells.argtypes = [C.c_int, C.c_int, C.c_double, _ip,
+                                np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS"), _dp]
+    L.or_elfs_cells(nrows, ncols, h, cstart, ext, dist)
+    n = nrows * ncols
+    return (ext[:n].reshape(ncols, nrows).T.astype(bool),
+            dist[:n].reshape(ncols, nrows).T)
 
 
 def ms_decide(prm: dict, dt, x, y, Ap, hp, vpx_dx, vpx_dy, vpy_dx, vpy_dy,
