@@ -39,7 +39,7 @@ __global__ void ms_pair_kernel(int nc, MsParams m, MsInputs in,
   for (int i = b; i < e; ++i) nsplit += act[cptcls[i]] == MS_SPLIT;
   const int formed =
       pair_cell(b, e, cptcls, in, m.max_dv2, act, partner, nullptr, 0);
-  const int undone = undo_cell(b, e, cptcls, m.min_ppc, act, partner);
+  const int undone = undo_cell(b, e, cptcls, m.min_ppc, 2, act, partner);
   if (nsplit) atomicAdd(&cnt[0], nsplit);
   if (formed) atomicAdd(&cnt[1], formed);
   if (formed - undone) atomicAdd(&cnt[2], formed - undone);
@@ -101,6 +101,10 @@ MsParams ms_params_from(const amsc_params &p) {
   m.min_ppc = p.ms_min_ppc;
   // (int)data.ms_max_ops, gpu_offload.cpp:317 (saturated instead of UB)
   m.max_ops = p.ms_max_ops >= 2147483647.0 ? 2147483647 : (int)p.ms_max_ops;
+  m.mode = p.ms_mode;
+  m.alpha = p.ms_alpha;
+  m.beta = p.ms_beta;
+  m.stretch = p.ms_stretch;
   return m;
 }
 
@@ -119,6 +123,11 @@ extern "C" int amsc_ms_decide(const amsc_params *p, int np, double dt,
                               int *d_partner, int *counts) {
   if (!p || np < 0 || p->nrows <= 0 || p->ncols <= 0 || !counts)
     return fail(AMSC_ERR_INVALID, "amsc_ms_decide: bad arguments");
+  if (p->ms_mode != 0)
+    return fail(AMSC_ERR_INVALID,
+                "amsc_ms_decide implements decide_actions of "
+                "merge_split_ops_cmes.h (ms_mode 0); the legacy ELFS rule "
+                "runs in the time loop");
   const MsParams m = ms_params_from(*p);
   MsInputs in;
   in.x = d_x;

@@ -1,5 +1,6 @@
-// merge_split.hpp -- device merge/split decisions (merge_split_ops_cmes.h):
-// the per-particle flag, the per-cell greedy pairing and the
+// merge_split.hpp -- device merge/split decisions (merge_split_ops_cmes.h,
+// and the legacy ELFS rule of merge_split_ops.h in the time loop): the
+// per-particle flag, the per-cell greedy pairing and the
 // min-particles-per-cell undo, shared by the standalone decision entry point
 // and the fused time loop.  Templated on the storage type R of the velocity,
 // gradient and area fields; the comparisons are evaluated in double.
@@ -21,7 +22,20 @@ enum : int {
 struct MsParams {
   double r_split, r_merge, shear_split, split_hp_min, hp_min, max_dv2;
   int min_level, max_level, min_ppc, max_ops;
+  // rule: 0 merge_split_ops_cmes.h, 1 the legacy ELFS rule of
+  // merge_split_ops.h (split where ELFS < alpha sqrt(Ap) or the divergence
+  // exceeds stretch, merge where ELFS > beta sqrt(Ap); a split makes four
+  // particles; at least two particles per cell)
+  int mode;
+  double alpha, beta, stretch;
 };
+
+__host__ __device__ __forceinline__ int split_weight(const MsParams &m) {
+  return m.mode == 1 ? 4 : 2;
+}
+__host__ __device__ __forceinline__ int cell_min_ppc(const MsParams &m) {
+  return m.mode == 1 ? 2 : m.min_ppc;  // merge_split_ops.h:478 fixes 2
+}
 
 template <typename R> struct MsIn {
   const double *x, *y, *hp;
@@ -109,11 +123,11 @@ __device__ int pair_cell(int b, int e, const int *cptcls, const MsIn<R> &in,
 // Undo merges (first primary in id order) until the cell keeps min_ppc
 // particles (merge_split_ops_cmes.h:230-249); returns the number undone.
 __device__ inline int undo_cell(int b, int e, const int *cptcls, int min_ppc,
-                                int *act, int *partner) {
+                                int split_w, int *act, int *partner) {
   int projected = 0;
   for (int i = b; i < e; ++i) {
     const int a = act[cptcls[i]];
-    projected += a == MS_KEEP ? 1 : a == MS_SPLIT ? 2 : a == MS_PRIMARY ? 1 : 0;
+    projected += a == MS_KEEP ? 1 : a == MS_SPLIT ? split_w : a == MS_PRIMARY ? 1 : 0;
   }
   int undone = 0;
   while (projected < min_ppc) {
@@ -137,15 +151,18 @@ __device__ inline int undo_cell(int b, int e, const int *cptcls, int min_ppc,
 }
 
 // The serial decision loop over cells [c0, c1) with the global caps
-// (merge_split_ops_cmes.h:176-249), from flags already set; cnt[0..2] =
-// splits, merges, merges.
+// (merge_split_ops_cmes.h:176-249, merge_split_ops.h:405-497), from flags
+// already set; cnt[0..2] = splits, merges, merges.  With early_stop the
+// loop ends after the first cell at which both caps are spent and cnt[3]
+// receives the first cell not visited (every later split and candidate is
+// then kept, which the caller applies); otherwise cnt[3] = c1.
 template <typename R>
 __device__ void ms_serial_cells(int c0, int c1, const MsParams &m,
                                 const MsIn<R> &in, const int *cstart,
                                 const int *cptcls, int *act, int *partner,
-                                int *cnt) {
-  int n_split = 0, n_merge = 0;
-  for (int cell = c0; cell < c1; ++cell) {
+                                int *cnt, bool early_stop = false) {
+  int n_split = 0, n_merge = 0, cell = c0;
+  for (; cell < c1; ++cell) {
     const int b = cstart[cell], e = cstart[cell + 1];
     if (b == e) continue;
     for (int i = b; i < e; ++i) {
@@ -156,11 +173,66 @@ __device__ void ms_serial_cells(int c0, int c1, const MsParams &m,
       }
     }
     pair_cell(b, e, cptcls, in, m.max_dv2, act, partner, &n_merge, m.max_ops);
-    n_merge -= undo_cell(b, e, cptcls, m.min_ppc, act, partner);
+    n_merge -= undo_cell(b, e, cptcls, cell_min_ppc(m), split_weight(m), act,
+                         partner);
+    // with both caps spent no later cell can split, pair or undo
+    if (early_stop && n_split >= m.max_ops && n_merge >= m.max_ops) {
+      ++cell;
+      break;
+    }
   }
   cnt[0] = n_split;
   cnt[1] = n_merge;
   cnt[2] = n_merge;
+  cnt[3] = cell;
+}
+
+// Per-cell inputs of the legacy ELFS rule: exterior cells and hop distances
+// to the fluid boundary (-1: not reached), indexed by global cell.
+struct ElfsCells {
+  const int *ext, *hops;
+};
+
+// compute_elfs (merge_split_ops.h:266-342) of one particle in local cell
+// `cell`: the cell distance hops * min(hx, hy), refined below max(hx, hy)
+// by the distance to the exterior cells of its row within two columns.
+__device__ __forceinline__ double elfs_value(const ElfsCells &E, int coff,
+                                             int nrows, int ncols, double hx,
+                                             double hy, int cell, double px,
+                                             double py) {
+  const double h = hx < hy ? hx : hy, thr = hx > hy ? hx : hy;
+  const int hop = E.hops[coff + cell];
+  const double base = hop < 0 ? 1.7976931348623157e308 : hop * h;
+  if (base > thr) return base;
+  double mind = base;
+  const int r0 = cell % nrows, c0 = cell / nrows;
+  for (int dc = -2; dc <= 2; ++dc) {
+    const int c = c0 + dc;
+    if (c < 0 || c >= ncols) continue;
+    if (!E.ext[coff + r0 + nrows * c]) continue;
+    const double cx_lo = c * hx, cx_hi = (c + 1) * hx;
+    const double cy_lo = r0 * hy, cy_hi = (r0 + 1) * hy;
+    const double nx = px < cx_lo ? cx_lo : (px > cx_hi ? cx_hi : px);
+    const double ny = py < cy_lo ? cy_lo : (py > cy_hi ? cy_hi : py);
+    const double dx = px - nx, dy = py - ny;
+    const double d = sqrt(dx * dx + dy * dy);
+    if (d < mind) mind = d;
+  }
+  return mind;
+}
+
+// Action flag of one particle under the ELFS rule, merge_split_ops.h:388-402.
+template <typename R>
+__device__ __forceinline__ int elfs_flag_one(const MsParams &m, double elfs,
+                                             const MsIn<R> &in, int ip) {
+  const double r_i = sqrt((double)in.Ap[ip]);
+  const double divergence = (double)in.vxdx[ip] + (double)in.vydy[ip];
+  const int lvl = in.level[ip];
+  if ((elfs < m.alpha * r_i) || (divergence > m.stretch))
+    return lvl > m.min_level ? MS_SPLIT : MS_KEEP;
+  if (elfs > m.beta * r_i && in.hp[ip] > m.hp_min)
+    return lvl < m.max_level ? MS_CANDIDATE : MS_KEEP;
+  return MS_KEEP;
 }
 
 __device__ __forceinline__ void principal_stretch_dir(double exx, double eyy,

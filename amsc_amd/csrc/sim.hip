@@ -105,6 +105,11 @@ struct Sim {
   P2GOut *p2g_out[2] = {nullptr, nullptr};
   int *act = nullptr, *partner = nullptr, *wts = nullptr, *offs = nullptr;
   int *ms_cnt = nullptr;
+  // legacy ELFS rule only (ms_mode 1): exterior flags, hop distances and
+  // two frontier queues, one int per cell each
+  int *elfs_ext = nullptr, *elfs_hops = nullptr, *elfs_qa = nullptr,
+      *elfs_qb = nullptr;
+  bool any_elfs = false;
   std::vector<int> ms_counts;  // splits, merges of the last call, per scenario
   std::vector<int> it_host;    // steps launched per scenario
   std::vector<char> ended;     // run known to have ended, per scenario
@@ -422,20 +427,35 @@ int merge_split(Sim &s, bool capture) {
   in.level = p.level;
   const int n = capture ? s.cap : s.np;
   const dim3 gp(grid_for(n, 256)), b256(256);
+  const int *key = s.key[s.kcur];
+  ElfsCells E{s.elfs_ext, s.elfs_hops};
   if (int rc = launch_fill_int(s, s.ms_cnt, 4ll * s.S, 0)) return rc;
+  if (s.any_elfs)
+    if (int rc = launch(s, "ms_elfs_cells", elfs_cells_kernel, dim3(s.S),
+                        dim3(1024), T, (const int *)s.bin.cstart, s.elfs_ext,
+                        s.elfs_hops, s.elfs_qa, s.elfs_qb))
+      return rc;
   if (n > 0)
-    if (int rc = launch(s, "ms_flag", ens_flag_kernel<R>, gp, b256, T, in,
-                        s.act, s.partner))
+    if (int rc = launch(s, "ms_flag", ens_flag_kernel<R>, gp, b256, T, in, E,
+                        key, s.act, s.partner))
       return rc;
   if (int rc = launch(s, "ms_pair", ens_pair_kernel<R>, dim3(grid_for(s.nc, 256)),
                       b256, T, (const int *)s.cell_off, s.nc, in,
                       (const int *)s.bin.cstart, (const int *)s.cptcls, s.act,
                       s.partner, s.ms_cnt))
     return rc;
+  if (n > 0)
+    if (int rc = launch(s, "ms_refresh", ens_refresh_kernel<R>, gp, b256, T,
+                        in, E, key, s.act, s.partner, (const int *)s.ms_cnt))
+      return rc;
   if (int rc = launch(s, "ms_serial", ens_serial_kernel<R>, dim3(s.S), dim3(64),
                       T, in, (const int *)s.bin.cstart, (const int *)s.cptcls,
                       s.act, s.partner, s.ms_cnt))
     return rc;
+  if (n > 0)
+    if (int rc = launch(s, "ms_demote", ens_demote_kernel, gp, b256, T, key,
+                        s.act, (const int *)s.ms_cnt))
+      return rc;
   if (!capture) {
     AMSC_HIP(hipMemcpyAsync(s.host_ints, s.ms_cnt, sizeof(int) * 4 * s.S,
                             hipMemcpyDeviceToHost, s.st));
@@ -451,9 +471,8 @@ int merge_split(Sim &s, bool capture) {
   {
     Timer tm(s, "ms_scan");
     if (n > 0)
-      if (int rc = launch(s, "ms_weights", weights_kernel, gp, b256,
-                          (const int *)s.poff, s.S, n, (const int *)s.act,
-                          s.wts))
+      if (int rc = launch(s, "ms_weights", weights_kernel, gp, b256, T, n,
+                          (const int *)s.act, s.wts))
         return rc;
     if (int rc = exclusive_scan(s.wts, s.offs, n, s.ms_scan, s.st, nullptr))
       return rc;
@@ -856,6 +875,10 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
     if (int rc = dalloc(*s, q, cap)) return rc;
   if (int rc = dalloc(*s, &s->offs, cap + 1)) return rc;
   if (int rc = dalloc(*s, &s->ms_cnt, 4 * S)) return rc;
+  for (int i = 0; i < S; ++i) s->any_elfs = s->any_elfs || prm[i].ms_mode == 1;
+  if (s->any_elfs)
+    for (int **q : {&s->elfs_ext, &s->elfs_hops, &s->elfs_qa, &s->elfs_qb})
+      if (int rc = dalloc(*s, q, std::max(s->nc, 1))) return rc;
   if (int rc = dalloc(*s, &s->poff, S + 1)) return rc;
   if (int rc = dalloc(*s, &s->poff_next, S + 1)) return rc;
   if (int rc = dalloc(*s, &s->wscen, cap / 64 + 1)) return rc;
@@ -1423,8 +1446,10 @@ long long bytes_estimate(int S, const amsc_params *prm, const int *np,
   const long long r = precision == AMSC_PREC_MIXED ? 4 : 8;
   const long long per_p = 2 * (4 * 8 + (PF_COUNT - 4) * r + 8) + 7 * 4 + 4;
   const long long per_n = 2 * r + (NF_COUNT - 2) * 8 + 8;
+  bool elfs = false;
+  for (int i = 0; i < S; ++i) elfs = elfs || prm[i].ms_mode == 1;
   // + upload staging: 3 FP64 values per initial particle
-  return cap * per_p + npt * 24 + (cap / 64 + 1) * 4 + nnt * per_n + nct * 8 +
+  return cap * per_p + npt * 24 + (cap / 64 + 1) * 4 + nnt * per_n + nct * (elfs ? 24 : 8) +
          ntl * (6 * 8 + 4 * kTileCfgs) +
          S * (long long)(sizeof(Ctrl) + kTileCfgs * sizeof(Scen) + 40) +
          (1 << 16);  // allocations are rounded up to 256 bytes

@@ -955,15 +955,133 @@ __global__ void __launch_bounds__(BS)
 }
 
 // ------------------------------------------------------------- merge/split
+// Fresh action flag of particle ip under its scenario's rule.
 template <typename R>
-__global__ void ens_flag_kernel(Tables T, MsIn<R> in, int *act, int *partner) {
+__device__ __forceinline__ int fresh_flag(const Scen &sc, const Ctrl &C,
+                                          const MsIn<R> &in, const ElfsCells &E,
+                                          const int *key, int ip) {
+  if (sc.ms.mode == 1) {
+    const double e = elfs_value(E, sc.coff, sc.P.nrows, sc.P.ncols, sc.P.hx,
+                                sc.P.hy, key[ip] - sc.coff, in.x[ip], in.y[ip]);
+    return elfs_flag_one(sc.ms, e, in, ip);
+  }
+  return ms_flag_one(sc.ms, C.dt, in, ip);
+}
+
+template <typename R>
+__global__ void ens_flag_kernel(Tables T, MsIn<R> in, ElfsCells E,
+                                const int *__restrict__ key, int *act,
+                                int *partner) {
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= T.poff[T.S]) return;
   const int sid = scen_of(T, ip);
   const Scen &sc = T.scen[sid];
   const Ctrl &C = T.ctrl[sid];
-  act[ip] = ms_due(sc, C) ? ms_flag_one(sc.ms, C.dt, in, ip) : MS_KEEP;
+  act[ip] = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, ip) : MS_KEEP;
   partner[ip] = -1;
+}
+
+// The two cell passes of the legacy ELFS rule (merge_split_ops.h:86-262) for
+// each scenario using it, one workgroup per scenario, level by level:
+//   ext[c] = 1 for the empty cells connected (4-neighbours, through empty
+//     cells) to an empty cell of the grid edge;
+//   hops[c] = fewest 4-neighbour steps through wet cells from a seed (a wet
+//     cell off the domain edge with an exterior cell left or right of it),
+//     -1 where no seed is reached.
+// Both are sets defined by reachability, so the order in which a level's
+// cells are claimed (atomics) does not change them.  qa, qb: frontier
+// queues; a cell enters each pass's queues once.
+__global__ void __launch_bounds__(1024)
+    elfs_cells_kernel(Tables T, const int *__restrict__ cstart, int *ext,
+                      int *hops, int *qa, int *qb) {
+  const int sid = blockIdx.x;
+  const Scen &sc = T.scen[sid];
+  if (sc.ms.mode != 1 || !ms_due(sc, T.ctrl[sid])) return;
+  const int nrows = sc.P.nrows, ncols = sc.P.ncols, nc = nrows * ncols;
+  const int *cs = cstart + sc.coff;
+  int *ex = ext + sc.coff, *hp = hops + sc.coff;
+  int *A = qa + sc.coff, *B = qb + sc.coff;
+  const int t = threadIdx.x, NT = blockDim.x;
+  __shared__ int s_n[2];
+  auto wet = [&](int c) { return cs[c + 1] > cs[c]; };
+  for (int i = t; i < nc; i += NT) {
+    ex[i] = 0;
+    hp[i] = -1;
+  }
+  if (t == 0) s_n[0] = 0;
+  __syncthreads();
+  // exterior: seeds are the empty cells of rows 0, nrows-1 and columns 0,
+  // ncols-1
+  const int perim = 2 * (nrows + ncols);
+  for (int i = t; i < perim; i += NT) {
+    int r, c;
+    if (i < 2 * ncols) {
+      c = i >> 1;
+      r = (i & 1) ? nrows - 1 : 0;
+    } else {
+      const int k = i - 2 * ncols;
+      r = k >> 1;
+      c = (k & 1) ? ncols - 1 : 0;
+    }
+    const int idx = r + nrows * c;
+    if (!wet(idx) && atomicCAS(&ex[idx], 0, 1) == 0) A[atomicAdd(&s_n[0], 1)] = idx;
+  }
+  __syncthreads();
+  int n = s_n[0];
+  int *cur = A, *nxt = B;
+  while (n > 0) {
+    __syncthreads();
+    if (t == 0) s_n[1] = 0;
+    __syncthreads();
+    for (int k = t; k < n; k += NT) {
+      const int cidx = cur[k], r = cidx % nrows, c = cidx / nrows;
+      const int nb[4] = {r > 0 ? cidx - 1 : -1, r < nrows - 1 ? cidx + 1 : -1,
+                         c > 0 ? cidx - nrows : -1, c < ncols - 1 ? cidx + nrows : -1};
+      for (int j = 0; j < 4; ++j)
+        if (nb[j] >= 0 && !wet(nb[j]) && atomicCAS(&ex[nb[j]], 0, 1) == 0)
+          nxt[atomicAdd(&s_n[1], 1)] = nb[j];
+    }
+    __syncthreads();
+    n = s_n[1];
+    int *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  // distance: seeds are wet cells off the domain edge with an exterior cell
+  // left or right of them
+  __syncthreads();
+  if (t == 0) s_n[0] = 0;
+  __syncthreads();
+  for (int i = t; i < nc; i += NT) {
+    const int r = i % nrows, c = i / nrows;
+    if (!wet(i) || r == 0 || r == nrows - 1 || c == 0 || c == ncols - 1) continue;
+    if (ex[i - nrows] || ex[i + nrows]) {
+      hp[i] = 0;
+      A[atomicAdd(&s_n[0], 1)] = i;
+    }
+  }
+  __syncthreads();
+  n = s_n[0];
+  cur = A;
+  nxt = B;
+  for (int level = 0; n > 0; ++level) {
+    __syncthreads();
+    if (t == 0) s_n[1] = 0;
+    __syncthreads();
+    for (int k = t; k < n; k += NT) {
+      const int cidx = cur[k], r = cidx % nrows, c = cidx / nrows;
+      const int nb[4] = {r > 0 ? cidx - 1 : -1, r < nrows - 1 ? cidx + 1 : -1,
+                         c > 0 ? cidx - nrows : -1, c < ncols - 1 ? cidx + nrows : -1};
+      for (int j = 0; j < 4; ++j)
+        if (nb[j] >= 0 && wet(nb[j]) && atomicCAS(&hp[nb[j]], -1, level + 1) == -1)
+          nxt[atomicAdd(&s_n[1], 1)] = nb[j];
+    }
+    __syncthreads();
+    n = s_n[1];
+    int *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
 }
 
 // One thread per cell: greedy pairing and the min-particles undo; counts
@@ -990,16 +1108,39 @@ __global__ void ens_pair_kernel(Tables T, const int *__restrict__ cell_off,
   for (int i = b; i < e; ++i) nsplit += act[cptcls[i]] == MS_SPLIT;
   const int formed =
       pair_cell(b, e, cptcls, in, sc.ms.max_dv2, act, partner, nullptr, 0);
-  const int undone = undo_cell(b, e, cptcls, sc.ms.min_ppc, act, partner);
+  const int undone = undo_cell(b, e, cptcls, cell_min_ppc(sc.ms),
+                               split_weight(sc.ms), act, partner);
   int *c = cnt + 4 * lo;
   if (nsplit) atomicAdd(&c[0], nsplit);
   if (formed) atomicAdd(&c[1], formed);
   if (formed - undone) atomicAdd(&c[2], formed - undone);
 }
 
-// One workgroup per scenario: where the operation cap could bind (more
-// splits or pairs than ms_max_ops), replay the exact serial loop from fresh
-// flags.
+// Where an operation cap could bind (more splits or pairs than ms_max_ops
+// in the parallel pass), the exact serial loop replaces the parallel
+// result: fresh flags for the scenario's particles (in parallel), the loop
+// over its cells in order until both caps are spent (one thread), and every
+// later split or candidate kept (in parallel).  cnt[4*s + 3] = the first
+// global cell the loop did not visit (0: no replay).
+__device__ __forceinline__ bool caps_bind(const Scen &sc, const Ctrl &C,
+                                          const int *c) {
+  return ms_due(sc, C) && (c[0] > sc.ms.max_ops || c[1] > sc.ms.max_ops);
+}
+
+template <typename R>
+__global__ void ens_refresh_kernel(Tables T, MsIn<R> in, ElfsCells E,
+                                   const int *__restrict__ key, int *act,
+                                   int *partner, const int *cnt) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= T.poff[T.S]) return;
+  const int sid = scen_of(T, ip);
+  const Scen &sc = T.scen[sid];
+  const Ctrl &C = T.ctrl[sid];
+  if (!caps_bind(sc, C, cnt + 4 * sid)) return;
+  act[ip] = fresh_flag(sc, C, in, E, key, ip);
+  partner[ip] = -1;
+}
+
 template <typename R>
 __global__ void ens_serial_kernel(Tables T, MsIn<R> in,
                                   const int *__restrict__ cstart,
@@ -1008,24 +1149,35 @@ __global__ void ens_serial_kernel(Tables T, MsIn<R> in,
   if (threadIdx.x != 0) return;
   const int sid = blockIdx.x;
   const Scen &sc = T.scen[sid];
-  const Ctrl &C = T.ctrl[sid];
-  if (!ms_due(sc, C)) return;
   int *c = cnt + 4 * sid;
-  if (!(c[0] > sc.ms.max_ops || c[1] > sc.ms.max_ops)) return;
-  for (int ip = T.poff[sid]; ip < T.poff[sid + 1]; ++ip) {
-    act[ip] = ms_flag_one(sc.ms, C.dt, in, ip);
-    partner[ip] = -1;
-  }
+  if (!caps_bind(sc, T.ctrl[sid], c)) return;
   ms_serial_cells(sc.coff, sc.coff + sc.P.nrows * sc.P.ncols, sc.ms, in,
-                  cstart, cptcls, act, partner, c);
+                  cstart, cptcls, act, partner, c, true);
 }
 
-__global__ void weights_kernel(const int *__restrict__ poff, int S, int n,
-                               const int *__restrict__ act, int *w) {
+__global__ void ens_demote_kernel(Tables T, const int *__restrict__ key,
+                                  int *act, const int *cnt) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= T.poff[T.S]) return;
+  const int stop = cnt[4 * scen_of(T, ip) + 3];
+  if (stop == 0 || key[ip] < stop) return;
+  const int a = act[ip];
+  if (a == MS_SPLIT || a == MS_CANDIDATE) act[ip] = MS_KEEP;
+}
+
+__global__ void weights_kernel(Tables T, int n, const int *__restrict__ act,
+                               int *w) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int a = i < poff[S] ? act[i] : -1;
-  w[i] = a == MS_KEEP ? 1 : a == MS_SPLIT ? 2 : a == MS_PRIMARY ? 1 : 0;
+  if (i >= T.poff[T.S]) {
+    w[i] = 0;
+    return;
+  }
+  const int a = act[i];
+  w[i] = a == MS_KEEP      ? 1
+         : a == MS_SPLIT   ? split_weight(T.scen[scen_of(T, i)].ms)
+         : a == MS_PRIMARY ? 1
+                           : 0;
 }
 
 // New particle offsets of every scenario (order is preserved, so scenario
@@ -1078,6 +1230,70 @@ __device__ __forceinline__ void stv(const PPtr &p, int k, int i, double v) {
   else fld<R>(p, k)[i] = (R)v;
 }
 
+// execute_merge_split of the legacy ELFS rule (merge_split_ops.h:603-710):
+// a split leaves four children at (x +- r/2, y +- r/2), r = sqrt(Ap),
+// clamped to the domain, extensive fields quartered, level - 1, label -1; a
+// merge averages the intensive fields by mass (depth included), sums the
+// extensive ones, keeps the primary's label, level + 1.  Momentum = Mp * v
+// and the stored friction force Ap * Fb follow the new fields.
+template <typename R>
+__device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
+                                 double Lx, double Ly) {
+  const PPtr &in = a.in, &out = a.out;
+  if (act == MS_SPLIT) {
+    const double px = ldv<R>(in, PF_X, i), py = ldv<R>(in, PF_Y, i);
+    const double off = sqrt(ldv<R>(in, PF_AP, i)) / 2.0;
+    double cx[4] = {px + off, px + off, px - off, px - off};
+    double cy[4] = {py + off, py - off, py + off, py - off};
+    for (int k = 0; k < 4; ++k) {
+      if (cx[k] < 1e-10 || cx[k] > Lx - 1e-10) {
+        const double m = Lx - 1e-10 < cx[k] ? Lx - 1e-10 : cx[k];
+        cx[k] = 1e-10 > m ? 1e-10 : m;
+      }
+      if (cy[k] < 1e-10 || cy[k] > Ly - 1e-10) {
+        const double m = Ly - 1e-10 < cy[k] ? Ly - 1e-10 : cy[k];
+        cy[k] = 1e-10 > m ? 1e-10 : m;
+      }
+    }
+    for (int c = 0; c < 4; ++c) {
+      const int oc = o + c;
+      for (int k = PF_VPX; k < PF_COUNT; ++k)
+        stv<R>(out, k, oc, ldv<R>(in, k, i) / (pf_extensive(k) ? 4.0 : 1.0));
+      stv<R>(out, PF_X, oc, cx[c]);
+      stv<R>(out, PF_Y, oc, cy[c]);
+      const double Mc = ldv<R>(out, PF_MP, oc);
+      stv<R>(out, PF_MOMX, oc, Mc * ldv<R>(out, PF_VPX, oc));
+      stv<R>(out, PF_MOMY, oc, Mc * ldv<R>(out, PF_VPY, oc));
+      const double Ap = ldv<R>(out, PF_AP, oc);
+      stv<R>(out, PF_FRX, oc, Ap * ldv<R>(out, PF_FBX, oc));
+      stv<R>(out, PF_FRY, oc, Ap * ldv<R>(out, PF_FBY, oc));
+      out.label[oc] = -1;
+      out.level[oc] = in.level[i] - 1;
+    }
+  } else {
+    const int j = a.partner[i];
+    const double M1 = ldv<R>(in, PF_MP, i), M2 = ldv<R>(in, PF_MP, j);
+    const double Mt = M1 + M2;
+    const double nx = (M1 * ldv<R>(in, PF_X, i) + M2 * ldv<R>(in, PF_X, j)) / Mt;
+    const double ny = (M1 * ldv<R>(in, PF_Y, i) + M2 * ldv<R>(in, PF_Y, j)) / Mt;
+    for (int k = PF_VPX; k < PF_COUNT; ++k)
+      stv<R>(out, k, o,
+             pf_extensive(k)
+                 ? (ldv<R>(in, k, i) + ldv<R>(in, k, j))
+                 : (M1 * ldv<R>(in, k, i) + M2 * ldv<R>(in, k, j)) / Mt);
+    stv<R>(out, PF_X, o, nx);
+    stv<R>(out, PF_Y, o, ny);
+    const double Mp = ldv<R>(out, PF_MP, o);
+    stv<R>(out, PF_MOMX, o, Mp * ldv<R>(out, PF_VPX, o));
+    stv<R>(out, PF_MOMY, o, Mp * ldv<R>(out, PF_VPY, o));
+    const double Ap = ldv<R>(out, PF_AP, o);
+    stv<R>(out, PF_FRX, o, Ap * ldv<R>(out, PF_FBX, o));
+    stv<R>(out, PF_FRY, o, Ap * ldv<R>(out, PF_FBY, o));
+    out.label[o] = in.label[i];
+    out.level[o] = in.level[i] + 1;
+  }
+}
+
 template <typename R>
 __global__ void ms_execute_kernel(ExecArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1097,6 +1313,10 @@ __global__ void ms_execute_kernel(ExecArgs a) {
   const Scen &sc = a.T.scen[scen_of(a.T, i)];
   const double Lx = sc.P.ncols * sc.P.hx, Ly = sc.P.nrows * sc.P.hy, eps = 1e-10;
   const PPtr &in = a.in, &out = a.out;
+  if (sc.ms.mode == 1) {
+    elfs_execute_one<R>(a, i, o, act, Lx, Ly);
+    return;
+  }
   if (act == MS_SPLIT) {
     const int o1 = o, o2 = o + 1;
     double ux, uy;
