@@ -112,6 +112,9 @@ struct Sim {
   bool any_elfs = false;
   std::vector<int> ms_counts;  // splits, merges of the last call, per scenario
   std::vector<int> it_host;    // steps launched per scenario
+  // per scenario: the stored friction force field is current (the last
+  // particle update wrote it, or no step has run since it was set)
+  std::vector<char> fric_cur;
   std::vector<char> ended;     // run known to have ended, per scenario
   // default tiling: the fastest on every workload measured (8x32-node P2G
   // tiles in linear order, 512-thread particle workgroups)
@@ -724,6 +727,7 @@ int launch_step(Sim &s) {
   for (int i = 0; i < s.S; ++i) {
     if (!s.ended[i]) {
       s.it_host[i]++;
+      s.fric_cur[i] = aux >= AUX_MS;
       s.particle_steps += s.poff_h[i + 1] - s.poff_h[i];
       const amsc_params &p = s.prm[i];
       const bool fixed = p.dt_fixed > 0.0;
@@ -863,6 +867,7 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   }
   s->ms_counts.assign(2 * S, 0);
   s->it_host.assign(S, 0);
+  s->fric_cur.assign(S, 1);
   s->ended.assign(S, 0);
 
   if (stream) s->st = stream;
@@ -1032,6 +1037,7 @@ int upload_ctrl(Sim *s, int first) {
     std::memset(&c, 0, sizeof(c));
     c.dt = 1.0e-3;  // gpu_offload.cpp:255
     c.active = 1;
+    c.fric_set = 1;  // the first P2G reads Fric_p = Ap * 0
   }
   if (c0.empty()) return AMSC_OK;
   AMSC_HIP(hipMemcpyAsync(s->ctrl + first, c0.data(), sizeof(Ctrl) * c0.size(),
@@ -1155,6 +1161,7 @@ int repack(Sim *o, Sim *s, int nkeep, const int *keep, int nadd,
       s->ended[i] = o->ended[k];
       s->ms_counts[2 * i] = o->ms_counts[2 * k];
       s->ms_counts[2 * i + 1] = o->ms_counts[2 * k + 1];
+      s->fric_cur[i] = o->fric_cur[k];
     }
   }
   if (nadd > 0) {
@@ -1209,6 +1216,23 @@ int copy_out(Sim &s, const void *src, size_t elem, size_t len, double *out) {
   return AMSC_OK;
 }
 
+// Make scenario sc's stored friction force current (see Ctrl::fric_set).
+int fric_materialize(Sim &s, int sc) {
+  if (s.fric_cur[sc]) return AMSC_OK;
+  const int np = s.poff_h[sc + 1] - s.poff_h[sc];
+  if (np > 0) {
+    const Tables T = tables(s);
+    const dim3 g(grid_for(np, 256)), b(256);
+    if (int rc = s.mixed ? launch(s, "fric", fric_kernel<float>, g, b, T,
+                                  s.buf[s.cur], sc)
+                         : launch(s, "fric", fric_kernel<double>, g, b, T,
+                                  s.buf[s.cur], sc))
+      return rc;
+  }
+  s.fric_cur[sc] = 1;
+  return AMSC_OK;
+}
+
 int get_field(Sim &s, int sc, const char *name, double *out, int n) {
   if (!name || !out || sc < 0 || sc >= s.S)
     return fail(AMSC_ERR_INVALID, "bad arguments");
@@ -1224,6 +1248,8 @@ int get_field(Sim &s, int sc, const char *name, double *out, int n) {
                                       "' was not written by the last step "
                                       "(enable amsc_sim_set_aux)");
     if ((size_t)n < np) return fail(AMSC_ERR_INVALID, "output buffer too small");
+    if (k == PF_FRX || k == PF_FRY)
+      if (int rc = fric_materialize(s, sc)) return rc;
     const size_t e = pf_bytes(s, k);
     return copy_out(s, static_cast<const char *>(s.buf[s.cur].f[k]) + e * p0,
                     e, np, out);
@@ -1280,6 +1306,14 @@ int set_field(Sim &s, int sc, const char *name, const double *in, int n) {
   const int p0 = s.poff_h[sc], np = s.poff_h[sc + 1] - p0;
   if (n != np) return fail(AMSC_ERR_INVALID, "length must equal num_particles");
   if (np == 0) return AMSC_OK;
+  // the next P2G reads the stored friction force as it was before the write
+  if (int rc = fric_materialize(s, sc)) return rc;
+  {
+    const int one = 1;
+    AMSC_HIP(hipMemcpyAsync(&s.ctrl[sc].fric_set, &one, sizeof(int),
+                            hipMemcpyHostToDevice, s.st));
+    AMSC_HIP(hipStreamSynchronize(s.st));
+  }
   const size_t e = pf_bytes(s, k);
   char *dst = static_cast<char *>(s.buf[s.cur].f[k]) + e * p0;
   if (e == 8) {

@@ -84,13 +84,21 @@ struct Ctrl {
   int it;          // steps taken
   int active;      // 0 once the run has ended
   int diverged;    // 1: a particle left the physical range (run stopped)
+  // The next P2G's friction force Fric_p (optim_par.cpp:456-459) is
+  // Ap * Fb of the last particle update, a function of the stored vpx, vpy,
+  // hp and Vp, which P2G recomputes instead of reading, except where the
+  // stored field holds something else: at creation (Fb = 0), after
+  // merge/split (mass-averaged Fb), after a field was overwritten.
+  // fric_set: the host asks the next step to read the stored field;
+  // fric_read: the prologue's copy of it for the step being taken.
+  int fric_set, fric_read;
   int pad;
   // max |vpx|, |vpy|, hp of the particles as bit patterns of non-negative
   // doubles, by step parity, spread over 64 addresses so that the per-block
   // atomics of the particle kernel do not all hit one line
   unsigned long long cfl[2][64][3];
 };
-constexpr size_t kCtrlHead = 56;  // bytes before cfl (what the host reads)
+constexpr size_t kCtrlHead = 64;  // bytes before cfl (what the host reads)
 static_assert(offsetof(Ctrl, cfl) == kCtrlHead, "control block header size");
 
 // Divergence stop (no counterpart in optim_par, whose loop would step on
@@ -223,6 +231,29 @@ __global__ void __launch_bounds__(64)
     ctrl->t_start = ctrl->t;
     ctrl->t = ctrl->t + dt;
     ctrl->it = s;
+    ctrl->fric_read = ctrl->fric_set;
+    ctrl->fric_set = 0;
+  }
+}
+
+// Voellmy basal friction Fb (optim_par.cpp:613-626) and area Ap
+// (optim_par.cpp:605-607) of a particle from its updated velocity, depth
+// and volume; Fric_p = Ap * Fb.
+template <typename A>
+__device__ __forceinline__ void voellmy(const KParams &P, A vpx, A vpy,
+                                        double h, A Vp, A &Fbx, A &Fby,
+                                        A &Ap) {
+  const A hR = (A)h;
+  Ap = h > 1.e-2 ? Vp / hR : A(0);
+  const A rho = (A)P.rho, g = (A)P.g;
+  Fbx = 0;
+  Fby = 0;
+  const A nv = sqrt(vpx * vpx + vpy * vpy);
+  if (P.fric > 0 && nv > A(1e-10) && P.xi > 0) {
+    const A v2 = vpx * vpx + vpy * vpy;
+    const A fric = (A)P.fric, tan_fa = (A)P.tan_fa, xi = (A)P.xi;
+    Fbx = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpx / nv;
+    Fby = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpy / nv;
   }
 }
 
@@ -270,6 +301,21 @@ __global__ void init_kernel(Tables T, PPtr p, const int *key,
   fld<R>(p, PF_MOMY)[ip] = (R)mom_y_in[ip - p_begin];
   p.label[ip] = ip - T.poff[sid];
   p.level[ip] = 0;
+}
+
+// Stored friction force Ap * Fb of the particles of scenario sid from their
+// stored vpx, vpy, hp and Vp: what the last particle update would have
+// written, for a reader of the field after a step that did not write it.
+template <typename R>
+__global__ void fric_kernel(Tables T, PPtr p, int sid) {
+  using A = Arith<R>;
+  const int ip = T.poff[sid] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= T.poff[sid + 1]) return;
+  A fbx, fby, ap;
+  voellmy<A>(T.scen[sid].P, (A)fld<R>(p, PF_VPX)[ip], (A)fld<R>(p, PF_VPY)[ip],
+             fld<double>(p, PF_HP)[ip], (A)fld<R>(p, PF_VP)[ip], fbx, fby, ap);
+  fld<R>(p, PF_FRX)[ip] = ap * fbx;
+  fld<R>(p, PF_FRY)[ip] = ap * fby;
 }
 
 // Cell key and arrival slot of every particle (re-binning after creation or
@@ -327,8 +373,11 @@ struct P2GIdx {
   int ip, s, j;
 };
 
+// fip: ip where the stored friction force is read, 0 where it is
+// recomputed (a load of one shared line, so that both cases issue the same
+// loads).
 template <typename R>
-__device__ __forceinline__ void p2g_fields(const P2GArgs &a, int ip,
+__device__ __forceinline__ void p2g_fields(const P2GArgs &a, int ip, int fip,
                                            P2GRaw<R> &W) {
   W.x = a.x[ip];
   W.y = a.y[ip];
@@ -338,8 +387,8 @@ __device__ __forceinline__ void p2g_fields(const P2GArgs &a, int ip,
   W.Vp = static_cast<const R *>(a.Vp)[ip];
   W.hp = a.hp[ip];
   W.H = a.H[ip];
-  W.frx = static_cast<const R *>(a.frx)[ip];
-  W.fry = static_cast<const R *>(a.fry)[ip];
+  W.frx = static_cast<const R *>(a.frx)[fip];
+  W.fry = static_cast<const R *>(a.fry)[fip];
   W.f11 = static_cast<const R *>(a.f11)[ip];
   W.f12 = static_cast<const R *>(a.f12)[ip];
   W.f22 = static_cast<const R *>(a.f22)[ip];
@@ -377,6 +426,9 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   const Scen &sc = a.T.scen[sid];
   Ctrl *ctrl = &a.T.ctrl[sid];
   if (!ctrl->active) return;
+  // stored friction force (first step, after merge/split or a field
+  // write) or Ap * Fb recomputed from the stored vpx, vpy, hp, Vp
+  const bool fric_read = ctrl->fric_read || ms_due(sc, *ctrl);
   const int nrows = sc.P.nrows, ncols = sc.P.ncols, bc = sc.P.bc;
   const double hx = sc.P.hx, hy = sc.P.hy, grav = sc.P.g, rho = sc.P.rho;
   const int coff = sc.coff, noff = sc.noff, toff = sc.toff, ntx = sc.ntx;
@@ -481,7 +533,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     build_own(CH);
     __syncthreads();
     fetch_idx(CH + t, I2);
-    p2g_fields<R>(a, I1.ip, W);
+    p2g_fields<R>(a, I1.ip, fric_read ? I1.ip : 0, W);
     __syncthreads();  // s_own is read above and rebuilt in the loop
     // Let the loads above land before the loop: otherwise the loop header
     // merges a path on which they are still in flight, and the compiler
@@ -526,8 +578,15 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
         s_n[0][ts] = W.Mp;
         s_n[1][ts] = (R)Fpx;
         s_n[2][ts] = (R)Fpy;
-        s_n[3][ts] = W.frx;
-        s_n[4][ts] = W.fry;
+        R frx = W.frx, fry = W.fry;
+        if (!fric_read) {
+          A fbx, fby, ap;
+          voellmy<A>(sc.P, (A)W.m1, (A)W.m2, W.hp, (A)W.Vp, fbx, fby, ap);
+          frx = ap * fbx;
+          fry = ap * fby;
+        }
+        s_n[3][ts] = frx;
+        s_n[4][ts] = fry;
         s_n[5][ts] = W.f11;
         s_n[6][ts] = W.f12;
         s_n[7][ts] = W.f22;
@@ -552,7 +611,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
       __syncthreads();
       // in flight during the fold: ids of chunk k+2, fields of chunk k+1
       fetch_idx(base + 2 * CH + t, I3);
-      p2g_fields<R>(a, I2.ip, W);
+      p2g_fields<R>(a, I2.ip, fric_read ? I2.ip : 0, W);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int lo = tb[k] > base ? tb[k] : base;
@@ -759,19 +818,10 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   h = h < 1e-2 ? 1e-2 : h;
   const A Vp = fld<R>(p, PF_VP)[ip];
   const A hR = (A)h;
-  const A Ap = h > 1.e-2 ? Vp / hR : A(0);
-  const A rho = (A)P.rho, g = (A)P.g;
-  // Voellmy basal friction, optim_par.cpp:613-626
-  A Fbx = 0, Fby = 0;
-  {
-    const A nv = sqrt(vpx * vpx + vpy * vpy);
-    if (P.fric > 0 && nv > A(1e-10) && P.xi > 0) {
-      const A v2 = vpx * vpx + vpy * vpy;
-      const A fric = (A)P.fric, tan_fa = (A)P.tan_fa, xi = (A)P.xi;
-      Fbx = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpx / nv;
-      Fby = -fric * (rho * g * hR * tan_fa + rho * g * v2 / xi) * vpy / nv;
-    }
-  }
+  A Fbx, Fby, Ap;
+  // from the velocity as stored, which is what a P2G that recomputes the
+  // friction force sees (the same value unless A is wider than R)
+  voellmy<A>(P, (A)(R)vpx, (A)(R)vpy, h, Vp, Fbx, Fby, Ap);
   // Bingham stress, stress_tensor_t::operator(), optim_par.cpp:68-120
   A F11, F12, F22;
   {
@@ -826,10 +876,11 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   fld<R>(p, PF_F11)[ip] = F11;
   fld<R>(p, PF_F12)[ip] = F12;
   fld<R>(p, PF_F22)[ip] = F22;
-  // friction force for the next P2G, optim_par.cpp:456-459
-  fld<R>(p, PF_FRX)[ip] = Ap * Fbx;
-  fld<R>(p, PF_FRY)[ip] = Ap * Fby;
   if (AUX >= AUX_MS) {
+    // friction force for a P2G that reads it (after merge/split),
+    // optim_par.cpp:456-459; other P2Gs recompute it
+    fld<R>(p, PF_FRX)[ip] = Ap * Fbx;
+    fld<R>(p, PF_FRY)[ip] = Ap * Fby;
     fld<R>(p, PF_AP)[ip] = Ap;
     fld<R>(p, PF_FBX)[ip] = Fbx;
     fld<R>(p, PF_FBY)[ip] = Fby;

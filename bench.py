@@ -53,14 +53,16 @@ HBM_PEAK_GBS = 8000.0
 HBM_BYTES = 288 * 2**30
 
 # Algorithmic bytes (FP64): fields each kernel must read or write once.
-#   p2g_nodal: reads x y Mp vpx vpy Vp hp H Fric_px Fric_py F11 F12 F22 per
-#              particle (104 B); per active node reads Z, writes vL (24 B).
+#   p2g_nodal: reads x y Mp vpx vpy Vp hp H F11 F12 F22 per particle (88 B;
+#              the friction force Ap*Fb is recomputed from vpx vpy hp Vp
+#              except on the steps that read it stored); per active node
+#              reads Z, writes vL (24 B).
 #   particle : reads x y hp Vp H (40 B), writes x y vpx vpy hp F11 F12 F22
-#              Fric_px Fric_py (80 B) per particle; per active node reads vL
-#              (16 B).
-#   whole step (state counted once per step): 184 B per particle + 120 B per
-#              active node.
-ALGO = {"p2g_nodal": (104, 24), "particle": (120, 16)}
+#              (64 B) per particle; per active node reads vL (16 B).
+#   whole step, the counting rule of the path (its state, Fric_p included,
+#              read and written once per step): 184 B per particle + 120 B
+#              per active node.
+ALGO = {"p2g_nodal": (88, 24), "particle": (104, 16)}
 STEP_ALGO = (184, 120)
 # Mixed layout: x y hp H FP64, the other 9 state fields FP32 (read 68 B,
 # write 52 B per particle); nodal vL FP32.

@@ -176,6 +176,48 @@ def test_merge_split_bitwise():
     assert splits > 50 and merges > 20, (splits, merges)
 
 
+def test_friction_force_recomputed_stored_and_rewritten():
+    """P2G recomputes the friction force Ap*Fb from the stored particle
+    state except after creation (Fb = 0), merge/split (mass-averaged Fb) and
+    a field write.  Reading Fric_px after a step that did not store it gives
+    that step's Ap*Fb (the oracle's Ap and Fb_x of the step, multiplied in
+    FP64); rewriting a field with its own values changes nothing; the run
+    stays the oracle's bit for bit through merge/split steps."""
+    from amsc_amd.simulation import Simulation
+    print("[sim] friction force", flush=True)
+    d = small_slope_case()
+    d.update(MERGE_SPLIT_ON=1, ms_call_interval=4, ms_alpha=0.3, ms_beta=0.52,
+             ms_max_dv=2.0, ms_hp_min=0.0)
+    prm = params_from_data(d)
+    part = particles_from_data(d)
+    o = oracle.OracleSim(prm, part, d["Z"])
+    g = Simulation(prm, part, d["Z"], capacity=8 * len(d["x"]))
+    h = Simulation(prm, part, d["Z"], capacity=8 * len(d["x"]))
+    assert not np.any(g.get("Fric_px"))  # Ap * 0 before the first step
+    for k in range(1, 15):
+        assert o.step()
+        g.step(1)
+        h.step(1)
+        if k in (2, 4, 5, 9):
+            fx = g.get("Fric_px")
+            assert np.array_equal(bits(fx), bits(o.get("Ap") * o.get("Fb_x"))), k
+            fy = g.get("Fric_py")
+            assert np.array_equal(bits(fy), bits(o.get("Ap") * o.get("Fb_y"))), k
+        if k in (3, 6):
+            g.set("vpx", g.get("vpx"))
+            g.set("hp", g.get("hp"))
+    g.set_aux(True)
+    h.set_aux(True)
+    o.step()
+    g.step(1)
+    h.step(1)
+    assert_same(o, g, "after 15 steps")
+    for k in PFIELDS + ("Fric_px", "Fric_py"):
+        assert np.array_equal(bits(g.get(k)), bits(h.get(k))), k
+    g.close()
+    h.close()
+
+
 def test_first_step_uses_data_momentum():
     """DATA momentum that is not vpx*Mp feeds the first P2G
     (optim_par.cpp:151-152,383); afterwards mom = vpx*Mp."""
