@@ -22,6 +22,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 OBJ_DIR = os.path.join(OUT_DIR, "obj")
 LIB = os.path.join(OUT_DIR, "libamsc_mpm.so")
 RUN = os.path.join(OUT_DIR, "amsc_run")
+OFFLOAD_LOOP = os.path.join(OUT_DIR, "amsc_offload_loop")
 
 HIP_SOURCES = ["runtime.hip", "dropin.hip", "binning.hip", "merge_split.hip",
                "sim.hip"]
@@ -81,11 +82,12 @@ def build(verbose: bool = False) -> str:
     subprocess.run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs],
                    check=True)
     os.replace(tmp, LIB)
-    subprocess.run([hipcc, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", INCLUDE,
-                    os.path.join(CSRC, "amsc_run.cpp"), "-o", RUN,
-                    f"-L{OUT_DIR}", "-lamsc_mpm", f"-Wl,-rpath,{OUT_DIR}",
-                    "-Wl,-rpath,$ORIGIN"],
-                   check=True)
+    for src, exe in (("amsc_run.cpp", RUN), ("offload_loop.cpp", OFFLOAD_LOOP)):
+        subprocess.run([hipcc, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", INCLUDE,
+                        os.path.join(CSRC, src), "-o", exe,
+                        f"-L{OUT_DIR}", "-lamsc_mpm", f"-Wl,-rpath,{OUT_DIR}",
+                        "-Wl,-rpath,$ORIGIN"],
+                       check=True)
     return LIB
 
 

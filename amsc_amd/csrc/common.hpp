@@ -9,6 +9,7 @@
 // FMA contraction disabled every value is bit-identical to the reference's.
 #pragma once
 
+#include <cmath>
 #include <hip/hip_runtime.h>
 
 #include "amsc_mpm.h"
@@ -107,9 +108,30 @@ __device__ __forceinline__ double shape_dy(const Local &L, int inode,
 // gpu_kernels.cpp:43-59 produce in every thread).
 struct Recip {
   double ix, nix, iy, niy;
+  // hx and hy are powers of two: u / h is then u * (1/h) exactly (the same
+  // single rounding of the same real number), without the division sequence
+  int pow2;
 };
 inline Recip make_recip(double hx, double hy) {
-  return Recip{1.0 / hx, -1.0 / hx, 1.0 / hy, -1.0 / hy};
+  int ex = 0, ey = 0;
+  const bool p2 = std::frexp(hx, &ex) == 0.5 && std::frexp(hy, &ey) == 0.5;
+  return Recip{1.0 / hx, -1.0 / hx, 1.0 / hy, -1.0 / hy, p2 ? 1 : 0};
+}
+
+// local_coords with the grid's reciprocals: the same values.
+__device__ __forceinline__ Local local_coords(double x, double y, int c,
+                                              int r, double hx, double hy,
+                                              const Recip &R) {
+  const double u = x - c * hx, v = y - r * hy;
+  Local L;
+  if (R.pow2) {
+    L.xi = u * R.ix;
+    L.eta = v * R.iy;
+  } else {
+    L.xi = u / hx;
+    L.eta = v / hy;
+  }
+  return L;
 }
 
 __device__ __forceinline__ double shape_dx(const Local &L, int inode,
@@ -200,6 +222,17 @@ __device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
 __device__ __forceinline__ int cell_of(double x, double y, int nrows,
                                        int ncols, double hx, double hy) {
   int c = (int)floor(x / hx), r = (int)floor(y / hy);
+  c = c < 0 ? 0 : (c > ncols - 1 ? ncols - 1 : c);
+  r = r < 0 ? 0 : (r > nrows - 1 ? nrows - 1 : r);
+  return r + c * nrows;
+}
+
+// cell_of with the grid's reciprocals (see Recip::pow2): the same cell.
+__device__ __forceinline__ int cell_of(double x, double y, int nrows,
+                                       int ncols, double hx, double hy,
+                                       const Recip &R) {
+  int c = (int)floor(R.pow2 ? x * R.ix : x / hx);
+  int r = (int)floor(R.pow2 ? y * R.iy : y / hy);
   c = c < 0 ? 0 : (c > ncols - 1 ? ncols - 1 : c);
   r = r < 0 ? 0 : (r > nrows - 1 ? nrows - 1 : r);
   return r + c * nrows;

@@ -331,7 +331,8 @@ __global__ void key_kernel(Tables T, const double *__restrict__ x,
   int k = 0;
   if (valid) {
     const Scen &S = T.scen[scen_of(T, ip)];
-    k = S.coff + cell_of(x[ip], y[ip], S.P.nrows, S.P.ncols, S.P.hx, S.P.hy);
+    k = S.coff + cell_of(x[ip], y[ip], S.P.nrows, S.P.ncols, S.P.hx, S.P.hy,
+                         S.rc);
   }
   const int sl = wave_slot(count, k, valid);
   if (valid) {
@@ -546,7 +547,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
       if (base + t < L) {
         // the particle's cell is the CSR cell it was found in
         const int pc = ccl + I1.s, pr = rl + I1.j;
-        const Local Lc = local_coords(W.x, W.y, pc, pr, hx, hy);
+        const Local Lc = local_coords(W.x, W.y, pc, pr, hx, hy, rc);
         // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
         const int zb = (pc - (c0 - 1)) * (TR + 2) + (pr - (r0 - 1));
         const double z0 = s_Z[zb], z1 = s_Z[zb + 1];
@@ -779,7 +780,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
     gy[k] = static_cast<const R *>(a.vvyL)[nid[k]];
   }
   const double x = fld<double>(p, PF_X)[ip], y = fld<double>(p, PF_Y)[ip];
-  const Local L0 = local_coords(x, y, c, r, hx, hy);
+  const Local L0 = local_coords(x, y, c, r, hx, hy, sc.rc);
   // g2p(vvxL, vvyL), optim_par.cpp:542
   const A xi0 = (A)L0.xi, eta0 = (A)L0.eta;
   A vxL = 0, vyL = 0;
@@ -799,7 +800,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   if (yn > Ly - eps) yn = Ly - eps;
   // g2pd(vvxL, vvyL) at the new position, step-start cell (D18),
   // optim_par.cpp:578
-  const Local L1 = local_coords(xn, yn, c, r, hx, hy);
+  const Local L1 = local_coords(xn, yn, c, r, hx, hy, sc.rc);
   const A xi1 = (A)L1.xi, eta1 = (A)L1.eta;
   const RecipT<A> rcr = recip_of<A>(sc);
   A vxdx = 0, vxdy = 0, vydx = 0, vydy = 0;
@@ -907,7 +908,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
     fld<R>(p, PF_MOMX)[ip] = vpx * Mp;
     fld<R>(p, PF_MOMY)[ip] = vpy * Mp;
   }
-  newkey = sc.coff + cell_of(xn, yn, P.nrows, P.ncols, hx, hy);
+  newkey = sc.coff + cell_of(xn, yn, P.nrows, P.ncols, hx, hy, sc.rc);
   vmx = fabs((double)vpx);
   vmy = fabs((double)vpy);
   hmx = h > 0.0 ? h : 0.0;

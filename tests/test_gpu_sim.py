@@ -140,10 +140,13 @@ def test_json_loader_matches_python_loader():
     b.close()
 
 
-@pytest.mark.parametrize("eq", [0.0, 2.0])
-def test_adaptive_dt_slope_bitwise(eq):
-    print(f"[sim] adaptive eq={eq}", flush=True)
-    d = small_slope_case(eq=eq)
+@pytest.mark.parametrize("eq,hx,hy", [(0.0, 0.5, 0.4), (2.0, 0.5, 0.4),
+                                       (0.0, 0.5, 0.25)])
+def test_adaptive_dt_slope_bitwise(eq, hx, hy):
+    """Grid spacings that are powers of two take the product with 1/h for
+    the local coordinates, the others the division: both bit for bit."""
+    print(f"[sim] adaptive eq={eq} h={hx}x{hy}", flush=True)
+    d = small_slope_case(eq=eq, hx=hx, hy=hy)
     d["DT_FIXED"] = 0.0
     d["CFL"] = 0.4
     d["T"] = 0.35

@@ -68,3 +68,22 @@ def test_scatter_colour_loop_touches_every_node_once_per_cell():
             for dr in (-1, 0, 1):
                 if (dc or dr) and 0 <= c + dc < ncols and 0 <= r + dr < nrows:
                     assert colour[cell] != colour[(r + dr) + (c + dc) * nrows]
+
+
+@pytest.mark.parametrize("case", ["slope", "dambreak"])
+def test_offload_loop_composition_matches_reference_build(case):
+    """gpu_offload's step (gpu_offload.cpp:410-562, fixed dt) composed from
+    the oracle's kernels equals the same composition over the reference's
+    own compiled kernels, bit for bit, at the end of the run: the loop the
+    device drop-in is checked against in test_gpu_offload_loop.py."""
+    if not oracle.ref_available():
+        pytest.skip("reference build not present (make -C oracle ref)")
+    from test_gpu_offload_loop import CASES, NODAL, PARTICLE, host_loop, initial_state
+    make, nsteps, dt = CASES[case]
+    d = make()
+    a = host_loop(initial_state(d, dt), nsteps, oracle.oracle_kernel)
+    b = host_loop(initial_state(d, dt), nsteps, oracle.ref_kernel)
+    for k in PARTICLE + NODAL:
+        assert np.array_equal(a[k].view(np.int64), b[k].view(np.int64)), k
+    for k in ("p2g", "cstart", "cptcls"):
+        assert np.array_equal(a[k], b[k]), k
