@@ -385,3 +385,40 @@ def test_served_requests_match_standalone_runs(graph):
         for f in ("x", "hp", "vpx"):
             assert np.array_equal(bits(s.get(f)), bits(r.result[f])), (r.rid, f)
         s.close()
+
+
+def test_empty_scenario_in_a_batch():
+    """A scenario without particles (a release that covers no cell) runs
+    beside the others through merge/split steps: it keeps zero particles and
+    the oracle's t and dt, the others stay their oracle's bit for bit; a
+    mixed batch and a single simulation without particles step as well."""
+    from amsc_amd.simulation import Ensemble, Simulation
+    print("[ens] empty scenario", flush=True)
+    a = small_slope_case()
+    b = small_slope_case(seed=9, nrows=14, ncols=11, hx=0.4, hy=0.5)
+    b.update(MERGE_SPLIT_ON=1, ms_call_interval=4, ms_alpha=0.3, ms_beta=0.52,
+             ms_max_dv=2.0, ms_hp_min=0.0)
+    prm, part, Z = scen(b)
+    empty = (prm, {k: v[:0] for k, v in part.items()}, Z)
+    cases = [scen(a), empty, scen(b)]
+    cap = 8 * sum(len(p["x"]) for _, p, _ in cases)
+    e = Ensemble(cases, "fp64", capacity=cap)
+    e.set_aux(True)
+    orc = [oracle.OracleSim(p, q, z) for p, q, z in cases]
+    for step in range(1, 13):
+        e.step(1)
+        for o in orc:
+            o.step()
+    for k, o in enumerate(orc):
+        assert_scen_equal(e, k, o, f"scenario {k}")
+    assert e.num(1) == 0 and not np.any(e.get(1, "vvxL"))
+    assert e.status()["np"][2] != len(part["x"])  # merge/split ran beside it
+    e.close()
+    m = Ensemble(cases, "mixed", capacity=cap)
+    assert m.step(12) == 12 and m.num(1) == 0
+    assert np.all(np.isfinite(m.get(0, "hp")))
+    m.close()
+    s = Simulation(*empty)
+    s.step(5)
+    assert s.num_particles == 0 and s.state()[0] == 5
+    s.close()
