@@ -507,7 +507,9 @@ int merge_split(Sim &s, bool capture) {
   ea.err = s.err;
   ea.ncopy = s.aux_on ? PF_COUNT : PF_CORE;
   if (n > 0)
-    if (int rc = launch(s, "ms_execute", ms_execute_kernel<R>, gp, b256, ea))
+    if (int rc = s.any_elfs
+                     ? launch(s, "ms_execute", ms_execute_kernel<R, true>, gp, b256, ea)
+                     : launch(s, "ms_execute", ms_execute_kernel<R, false>, gp, b256, ea))
       return rc;
   if (int rc = launch(s, "ms_commit", ms_commit_kernel, dim3(1), b256, s.poff,
                       (const int *)s.poff_next, s.S, (const int *)s.err))

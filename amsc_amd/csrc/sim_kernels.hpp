@@ -924,7 +924,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
 // the scenario up per particle and adds the maxima per particle.
 template <typename R, int AUX, int BS, bool EDGE>
 __global__ void __launch_bounds__(BS)
-    __attribute__((amdgpu_waves_per_eu(EDGE ? 1 : AUX == AUX_FULL ? 4 : 6)))
+    __attribute__((amdgpu_waves_per_eu(EDGE ? 1 : AUX == AUX_FULL ? 4 : AUX == AUX_MS ? 5 : 6)))
     particle_kernel(PartArgs a) {
   const Tables &T = a.T;
   const int np = T.poff[T.S];
@@ -1307,20 +1307,27 @@ __device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
         cy[k] = 1e-10 > m ? 1e-10 : m;
       }
     }
+    // field by field (the four children share every value but x, y)
+    for (int k = PF_VPX; k < PF_COUNT; ++k) {
+      const double v = ldv<R>(in, k, i) / (pf_extensive(k) ? 4.0 : 1.0);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) stv<R>(out, k, o + c, v);
+    }
+    const double Mc = ldv<R>(out, PF_MP, o), Ac = ldv<R>(out, PF_AP, o);
+    const double mx = Mc * ldv<R>(out, PF_VPX, o), my = Mc * ldv<R>(out, PF_VPY, o);
+    const double fx = Ac * ldv<R>(out, PF_FBX, o), fy = Ac * ldv<R>(out, PF_FBY, o);
+    const int lvl = in.level[i] - 1;
+#pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int oc = o + c;
-      for (int k = PF_VPX; k < PF_COUNT; ++k)
-        stv<R>(out, k, oc, ldv<R>(in, k, i) / (pf_extensive(k) ? 4.0 : 1.0));
       stv<R>(out, PF_X, oc, cx[c]);
       stv<R>(out, PF_Y, oc, cy[c]);
-      const double Mc = ldv<R>(out, PF_MP, oc);
-      stv<R>(out, PF_MOMX, oc, Mc * ldv<R>(out, PF_VPX, oc));
-      stv<R>(out, PF_MOMY, oc, Mc * ldv<R>(out, PF_VPY, oc));
-      const double Ap = ldv<R>(out, PF_AP, oc);
-      stv<R>(out, PF_FRX, oc, Ap * ldv<R>(out, PF_FBX, oc));
-      stv<R>(out, PF_FRY, oc, Ap * ldv<R>(out, PF_FBY, oc));
+      stv<R>(out, PF_MOMX, oc, mx);
+      stv<R>(out, PF_MOMY, oc, my);
+      stv<R>(out, PF_FRX, oc, fx);
+      stv<R>(out, PF_FRY, oc, fy);
       out.label[oc] = -1;
-      out.level[oc] = in.level[i] - 1;
+      out.level[oc] = lvl;
     }
   } else {
     const int j = a.partner[i];
@@ -1346,7 +1353,9 @@ __device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
   }
 }
 
-template <typename R>
+// ELFS: some scenario of the batch uses the legacy ELFS rule (its code is
+// compiled only into that variant).
+template <typename R, bool ELFS>
 __global__ void ms_execute_kernel(ExecArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (*a.err || i >= a.T.poff[a.T.S]) return;
@@ -1365,7 +1374,7 @@ __global__ void ms_execute_kernel(ExecArgs a) {
   const Scen &sc = a.T.scen[scen_of(a.T, i)];
   const double Lx = sc.P.ncols * sc.P.hx, Ly = sc.P.nrows * sc.P.hy, eps = 1e-10;
   const PPtr &in = a.in, &out = a.out;
-  if (sc.ms.mode == 1) {
+  if (ELFS && sc.ms.mode == 1) {
     elfs_execute_one<R>(a, i, o, act, Lx, Ly);
     return;
   }
