@@ -5,17 +5,20 @@
 // Pipeline (all integer, so results are exact and run-to-run identical):
 //   1. key = cell of the particle, slot = atomicAdd(count[key]) aggregated
 //      per wavefront (fused into the particle-update kernel of a time step);
-//   2. cstart[1..nc] = inclusive scan of count (one single-pass decoupled
-//      look-back scan, rocPRIM), cstart[0] = 0 from allocation;
+//   2. cstart = exclusive scan of count (reduce-then-scan: tile sums, one
+//      workgroup scans the tile sums, every tile scans its values from its
+//      offset; no look-back state to initialise, no waits between
+//      workgroups), the counts reset to 0 for the next binning as they are
+//      read;
 //   3. cptcls[cstart[key] + slot] = id;
 //   4. every cell's slice of cptcls sorted ascending: atomics fix which
 //      slots a cell receives but not their order, so this step restores the
-//      stable order of a CPU counting sort (ascending id within a cell);
-//      the cell's count is reset to 0 for the next binning.
+//      stable order of a CPU counting sort (ascending id within a cell).
 #include "binning.hpp"
 #include "amsc_mpm.h"
 
-#include <rocprim/device/device_scan.hpp>
+#include <climits>
+#include <cstdint>
 
 namespace amsc {
 
@@ -48,9 +51,7 @@ __global__ void place_kernel(int np, const int *__restrict__ np_dev,
   cptcls[cstart[key[ip]] + slot[ip]] = ip;
 }
 
-constexpr int kSortLocal = 32;
-
-// Heap sort of a[0..n) in place (cells above kSortLocal particles: O(n log n)
+// Heap sort of a[0..n) in place (cells above 8 particles: O(n log n)
 // however many particles pile into one cell).
 __device__ void heap_sort(int *a, int n) {
   auto sift = [a](int root, int end) {
@@ -75,36 +76,140 @@ __device__ void heap_sort(int *a, int n) {
   }
 }
 
+// Compare-exchange of a sorting network (constant indices: registers).
+__device__ __forceinline__ void cx(int &a, int &b) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
 __global__ void cell_sort_kernel(int nc, const int *__restrict__ cstart,
-                                 int *cptcls, int *count,
-                                 const int *__restrict__ active) {
+                                 int *cptcls, const int *__restrict__ active) {
   if (active && !*active) return;
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= nc) return;
-  count[cell] = 0;
   const int b = cstart[cell], e = cstart[cell + 1], n = e - b;
   if (n < 2) return;
-  if (n <= kSortLocal) {
-    int v[kSortLocal];
+  if (n <= 8) {
+    // the common case (a few particles per cell): Batcher's odd-even merge
+    // network for 8 keys (19 compare-exchanges), padded with INT_MAX
+    int v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = i < n ? cptcls[b + i] : INT_MAX;
     bool sorted = true;
-    for (int i = 0; i < n; ++i) {
-      v[i] = cptcls[b + i];
-      if (i > 0 && v[i] < v[i - 1]) sorted = false;
-    }
+#pragma unroll
+    for (int i = 1; i < 8; ++i) sorted = sorted && v[i - 1] <= v[i];
     if (sorted) return;
-    for (int i = 1; i < n; ++i) {
-      const int t = v[i];
-      int j = i - 1;
-      while (j >= 0 && v[j] > t) {
-        v[j + 1] = v[j];
-        --j;
-      }
-      v[j + 1] = t;
-    }
-    for (int i = 0; i < n; ++i) cptcls[b + i] = v[i];
+    cx(v[0], v[1]); cx(v[2], v[3]); cx(v[4], v[5]); cx(v[6], v[7]);
+    cx(v[0], v[2]); cx(v[1], v[3]); cx(v[4], v[6]); cx(v[5], v[7]);
+    cx(v[1], v[2]); cx(v[5], v[6]);
+    cx(v[0], v[4]); cx(v[1], v[5]); cx(v[2], v[6]); cx(v[3], v[7]);
+    cx(v[2], v[4]); cx(v[3], v[5]);
+    cx(v[1], v[2]); cx(v[3], v[4]); cx(v[5], v[6]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (i < n) cptcls[b + i] = v[i];
   } else {
     heap_sort(cptcls + b, n);
   }
+}
+
+// ---- reduce-then-scan of int counts (exact): 4096 values per tile, four
+// consecutive values per thread of a 1024-thread workgroup.
+constexpr int kScanThreads = 1024, kScanItems = 4,
+              kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ void load4(const int *in, long long base, int n,
+                                      int v[4]) {
+  if (base + 3 < n && (reinterpret_cast<uintptr_t>(in + base) & 15) == 0) {
+    const int4 q = *reinterpret_cast<const int4 *>(in + base);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = base + k < n ? in[base + k] : 0;
+  }
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over the workgroup (blockDim.x a
+// multiple of 64, at most 1024); *total = the workgroup's sum.  s_w holds 17
+// ints; the caller synchronises before reusing it.
+__device__ __forceinline__ int block_excl_scan(int v, int *s_w, int *total) {
+  const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int inc = wave_incl_scan(v);
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    const int t = lane < nw ? s_w[lane] : 0;
+    const int ti = wave_incl_scan(t);
+    if (lane < nw) s_w[lane] = ti - t;
+    if (lane == nw - 1) s_w[16] = ti;
+  }
+  __syncthreads();
+  *total = s_w[16];
+  return s_w[w] + inc - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+    scan_sums_kernel(const int *__restrict__ in, int n, int *tile_sum) {
+  __shared__ int s_w[17];
+  int v[4];
+  load4(in, (long long)blockIdx.x * kScanTile + threadIdx.x * kScanItems, n, v);
+  int total;
+  (void)block_excl_scan(v[0] + v[1] + v[2] + v[3], s_w, &total);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// one workgroup: tile_sum[0..nt) becomes its exclusive prefix
+__global__ void __launch_bounds__(kScanThreads)
+    scan_tile_sums_kernel(int *tile_sum, int nt) {
+  __shared__ int s_w[17];
+  int carry = 0;
+  for (int t0 = 0; t0 < nt; t0 += kScanThreads) {
+    const int i = t0 + threadIdx.x;
+    const int v = i < nt ? tile_sum[i] : 0;
+    int total;
+    const int ex = block_excl_scan(v, s_w, &total);
+    if (i < nt) tile_sum[i] = carry + ex;
+    carry += total;
+    __syncthreads();
+  }
+}
+
+// out[i+1] = prefix through in[i]; out[0] = 0; clear != null: clear[i] = 0
+// after in[i] is read (clear == in for the binning counts).
+__global__ void __launch_bounds__(kScanThreads)
+    scan_apply_kernel(const int *in, int n, const int *__restrict__ tile_off,
+                      int *out, int *clear) {
+  __shared__ int s_w[17];
+  const long long base =
+      (long long)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[4];
+  load4(in, base, n, v);
+  int total;
+  const int ex = block_excl_scan(v[0] + v[1] + v[2] + v[3], s_w, &total);
+  int run = tile_off[blockIdx.x] + ex;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) {
+      run += v[k];
+      out[base + k + 1] = run;
+      if (clear) clear[base + k] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
 }
 
 __global__ void colouring_kernel(int nrows, int ncols, int *ccidx, int o1,
@@ -121,14 +226,26 @@ __global__ void colouring_kernel(int nrows, int ncols, int *ccidx, int o1,
 
 }  // namespace
 
+namespace {
+int scan_tiles(int n) { return (int)(((long long)n + kScanTile - 1) / kScanTile); }
+
+int scan_launch(const int *in, int *out, int n, ScanWork &w, hipStream_t st,
+                int *clear) {
+  if (n <= 0) return AMSC_OK;
+  if (int rc = scan_reserve(w, n)) return rc;
+  const int nt = scan_tiles(n);
+  int *ts = static_cast<int *>(w.tmp);
+  scan_sums_kernel<<<nt, kScanThreads, 0, st>>>(in, n, ts);
+  scan_tile_sums_kernel<<<1, kScanThreads, 0, st>>>(ts, nt);
+  scan_apply_kernel<<<nt, kScanThreads, 0, st>>>(in, n, ts, out, clear);
+  AMSC_LAUNCH_CHECK();
+  return AMSC_OK;
+}
+}  // namespace
+
 int scan_bytes(int n, size_t *bytes) {
-  *bytes = 0;
-  const hipError_t e = rocprim::inclusive_scan(
-      nullptr, *bytes, static_cast<const int *>(nullptr),
-      static_cast<int *>(nullptr), (size_t)(n > 0 ? n : 1),
-      rocprim::plus<int>());
-  if (e != hipSuccess)
-    return fail(AMSC_ERR_HIP, std::string("scan sizing: ") + hipGetErrorString(e));
+  const size_t nt = (size_t)(n > 0 ? scan_tiles(n) : 1);
+  *bytes = (nt * sizeof(int) + 255) / 256 * 256;
   return AMSC_OK;
 }
 
@@ -149,15 +266,7 @@ int scan_reserve(ScanWork &w, int n) {
 
 int exclusive_scan(const int *in, int *out, int n, ScanWork &w,
                    hipStream_t st, int *total_host) {
-  if (n > 0) {
-    if (int rc = scan_reserve(w, n)) return rc;
-    size_t bytes = w.bytes;
-    const hipError_t e = rocprim::inclusive_scan(w.tmp, bytes, in, out + 1,
-                                                 (size_t)n, rocprim::plus<int>(),
-                                                 st);
-    if (e != hipSuccess)
-      return fail(AMSC_ERR_HIP, std::string("scan: ") + hipGetErrorString(e));
-  }
+  if (int rc = scan_launch(in, out, n, w, st, nullptr)) return rc;
   if (total_host) {
     AMSC_HIP(hipMemcpyAsync(total_host, out + n, sizeof(int),
                             hipMemcpyDeviceToHost, st));
@@ -200,13 +309,13 @@ int launch_finish_binning(Binning &b, int np, const int *key,
                           hipStream_t st, const int *np_dev) {
   const int nc = b.nrows * b.ncols;
   if (active) return fail(AMSC_ERR_INVALID, "binning: unsupported gating");
-  if (int rc = exclusive_scan(b.count, b.cstart, nc, b.scan, st, nullptr))
+  if (int rc = scan_launch(b.count, b.cstart, nc, b.scan, st, b.count))
     return rc;
   if (np > 0)
     place_kernel<<<grid_for(np, 256), 256, 0, st>>>(np, np_dev, key, slot,
                                                     b.cstart, cptcls, active);
   cell_sort_kernel<<<grid_for(nc, 256), 256, 0, st>>>(nc, b.cstart, cptcls,
-                                                      b.count, active);
+                                                      active);
   AMSC_LAUNCH_CHECK();
   return AMSC_OK;
 }

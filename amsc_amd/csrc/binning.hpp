@@ -5,7 +5,7 @@
 
 namespace amsc {
 
-// Device workspace of the single-pass scan.
+// Device workspace of the scan (one int per 4096-value tile).
 struct ScanWork {
   void *tmp = nullptr;
   size_t bytes = 0;
@@ -15,10 +15,9 @@ struct ScanWork {
 // Workspace bytes of a scan of n values.
 int scan_bytes(int n, size_t *bytes);
 
-// out[1..n] = inclusive prefix of in[0..n); out[0] is not written and must
-// already hold 0, so out[0..n] is the exclusive prefix with the total in
-// out[n].  If total_host != nullptr the total is copied back (synchronising
-// the stream).
+// out[0..n] = exclusive prefix of in[0..n) (out[0] = 0, the total in
+// out[n]; nothing is written for n = 0).  If total_host != nullptr the total
+// is copied back (synchronising the stream).
 int exclusive_scan(const int *in, int *out, int n, ScanWork &w,
                    hipStream_t st, int *total_host);
 
@@ -30,7 +29,7 @@ int scan_reserve(ScanWork &w, int n);
 struct Binning {
   int nrows = 0, ncols = 0;
   int *count = nullptr;   // [nc] cell counts (kept zero between uses)
-  int *cstart = nullptr;  // [nc+1], cstart[0] = 0 from allocation
+  int *cstart = nullptr;  // [nc+1], written by each binning
   ScanWork scan;
 };
 
