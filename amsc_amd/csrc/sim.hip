@@ -459,18 +459,6 @@ int merge_split(Sim &s, bool capture) {
     if (int rc = launch(s, "ms_demote", ens_demote_kernel, gp, b256, T, key,
                         s.act, (const int *)s.ms_cnt))
       return rc;
-  if (!capture) {
-    AMSC_HIP(hipMemcpyAsync(s.host_ints, s.ms_cnt, sizeof(int) * 4 * s.S,
-                            hipMemcpyDeviceToHost, s.st));
-    AMSC_HIP(hipStreamSynchronize(s.st));
-    bool any = false;
-    for (int i = 0; i < s.S; ++i) {
-      s.ms_counts[2 * i] = s.host_ints[4 * i];
-      s.ms_counts[2 * i + 1] = s.host_ints[4 * i + 2];
-      any = any || s.host_ints[4 * i] || s.host_ints[4 * i + 2];
-    }
-    if (!any) return AMSC_OK;  // no operation anywhere: nothing moves
-  }
   {
     Timer tm(s, "ms_scan");
     if (n > 0)
@@ -485,15 +473,27 @@ int merge_split(Sim &s, bool capture) {
                       s.poff_next, s.err))
     return rc;
   if (!capture) {
-    AMSC_HIP(hipMemcpyAsync(s.host_ints, s.err, sizeof(int),
+    // one read-back: operation counts, overflow flag, new offsets
+    int *h = s.host_ints;
+    AMSC_HIP(hipMemcpyAsync(h, s.ms_cnt, sizeof(int) * 4 * s.S,
                             hipMemcpyDeviceToHost, s.st));
-    AMSC_HIP(hipMemcpyAsync(s.host_ints + 1, s.poff_next + s.S, sizeof(int),
+    AMSC_HIP(hipMemcpyAsync(h + 4 * s.S, s.err, sizeof(int),
                             hipMemcpyDeviceToHost, s.st));
+    AMSC_HIP(hipMemcpyAsync(h + 4 * s.S + 1, s.poff_next,
+                            sizeof(int) * (s.S + 1), hipMemcpyDeviceToHost,
+                            s.st));
     AMSC_HIP(hipStreamSynchronize(s.st));
-    if (s.host_ints[0]) {
+    bool any = false;
+    for (int i = 0; i < s.S; ++i) {
+      s.ms_counts[2 * i] = h[4 * i];
+      s.ms_counts[2 * i + 1] = h[4 * i + 2];
+      any = any || h[4 * i] || h[4 * i + 2];
+    }
+    if (!any) return AMSC_OK;  // no operation anywhere: nothing moves
+    if (h[4 * s.S]) {
       (void)launch_fill_int(s, s.err, 1, 0);
       return fail(AMSC_ERR_CAPACITY,
-                  "merge/split needs " + std::to_string(s.host_ints[1]) +
+                  "merge/split needs " + std::to_string(h[5 * s.S + 1]) +
                       " particles, capacity is " + std::to_string(s.cap));
     }
   }
@@ -516,8 +516,12 @@ int merge_split(Sim &s, bool capture) {
     return rc;
   if (int rc = update_wscen(s)) return rc;
   s.cur = 1 - s.cur;
-  if (!capture)
-    if (int rc = read_poff(s)) return rc;
+  if (!capture) {
+    // the committed offsets, read back above
+    const int *h = s.host_ints + 4 * s.S + 1;
+    s.poff_h.assign(h, h + s.S + 1);
+    s.np = s.poff_h[s.S];
+  }
   return rebin(s, capture);
 }
 
