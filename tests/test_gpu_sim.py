@@ -140,6 +140,37 @@ def test_json_loader_matches_python_loader():
     b.close()
 
 
+def test_json_loader_reads_every_parameter(tmp_path):
+    """The C DATA.json reader and the Python loader agree on every solver
+    parameter (amsc_sim_params), the merge/split rule selection included,
+    for the reference's own DATA_bench_1000.json and for a copy with every
+    optional key set away from its default."""
+    import json
+    from amsc_amd.simulation import Simulation
+    from amsc_amd import native
+    import ctypes as C
+    with open(os.path.join(GOLDEN, "DATA_bench_1000.json")) as f:
+        raw = json.load(f)
+    alt = dict(raw, ms_mode=1, ms_stretch_thresh=0.75, ms_alpha=1.7, ms_beta=2.9,
+               ms_max_ops=11, ms_call_interval=3, ms_min_level=-2, ms_max_level=3,
+               ms_min_particles_per_cell=3, MERGE_SPLIT_ON=1, eq_level=1.5, CFL=0.3)
+    for name, data in (("reference", raw), ("optional keys", alt)):
+        path = tmp_path / f"{name.replace(' ', '_')}.json"
+        with open(path, "w") as f:
+            json.dump(data, f)
+        d = load_data_json(str(path))
+        h = C.c_void_p()
+        native.check(native.lib().amsc_sim_create_from_json(str(path).encode(), 0, 0,
+                                                            C.byref(h)))
+        a = Simulation.__new__(Simulation)
+        a._h = h
+        got = a.params()
+        want = params_from_data(d)
+        for k, v in got.items():
+            assert v == want[k], (name, k, v, want[k])
+        a.close()
+
+
 @pytest.mark.parametrize("eq,hx,hy", [(0.0, 0.5, 0.4), (2.0, 0.5, 0.4),
                                        (0.0, 0.5, 0.25)])
 def test_adaptive_dt_slope_bitwise(eq, hx, hy):
