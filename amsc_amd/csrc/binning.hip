@@ -19,6 +19,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <mutex>
 
 namespace amsc {
 
@@ -343,6 +344,67 @@ int launch_colouring(int nrows, int ncols, int *ccidx, int *coloff,
 
 using namespace amsc;
 
+namespace {
+
+// Workspace of amsc_bin_particles kept between calls (a time loop bins every
+// step): the cell counts (zero between calls: the scan clears them), the
+// scan's tile sums and the arrival slots, grown as needed, per device.  It
+// is held until the process ends (never freed at exit, when the HIP runtime
+// may already be gone).
+struct BinCache {
+  int dev = -1, nc_cap = 0, np_cap = 0;
+  int *count = nullptr, *slot = nullptr;
+  ScanWork scan;
+};
+
+std::mutex &bin_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+BinCache &bin_cache() {
+  static BinCache *c = new BinCache;
+  return *c;
+}
+
+void bin_cache_drop(BinCache &c) {
+  if (c.count) (void)hipFree(c.count);
+  if (c.slot) (void)hipFree(c.slot);
+  if (c.scan.tmp) (void)hipFree(c.scan.tmp);
+  c = BinCache();
+}
+
+int bin_cache_reserve(BinCache &c, int nc, int np) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return fail(AMSC_ERR_HIP, "amsc_bin_particles: no device");
+  if (dev != c.dev) bin_cache_drop(c);
+  c.dev = dev;
+  if (nc > c.nc_cap) {
+    if (c.count) (void)hipFree(c.count);
+    c.count = nullptr;
+    c.nc_cap = 0;
+    const int cap = nc + nc / 2;
+    if (hipMalloc(&c.count, sizeof(int) * cap) != hipSuccess)
+      return fail(AMSC_ERR_NOMEM, "amsc_bin_particles: count allocation");
+    if (hipMemset(c.count, 0, sizeof(int) * cap) != hipSuccess)
+      return fail(AMSC_ERR_HIP, "amsc_bin_particles: count clear");
+    c.nc_cap = cap;
+  }
+  if (np > c.np_cap) {
+    if (c.slot) (void)hipFree(c.slot);
+    c.slot = nullptr;
+    c.np_cap = 0;
+    const int cap = np + np / 2;
+    if (hipMalloc(&c.slot, sizeof(int) * cap) != hipSuccess)
+      return fail(AMSC_ERR_NOMEM, "amsc_bin_particles: slot allocation");
+    c.np_cap = cap;
+  }
+  return scan_reserve(c.scan, nc);
+}
+
+}  // namespace
+
 extern "C" int amsc_bin_particles(int np, int nrows, int ncols, double hx,
                                   double hy, const double *d_x,
                                   const double *d_y, int *d_p2g,
@@ -355,30 +417,26 @@ extern "C" int amsc_bin_particles(int np, int nrows, int ncols, double hx,
   if (!d_p2g || !d_cstart || (np > 0 && !d_cptcls))
     return fail(AMSC_ERR_INVALID, "amsc_bin_particles: NULL output");
   hipStream_t st = current_stream();
-  Binning b;
-  if (int rc = binning_alloc(b, nrows, ncols)) {
-    binning_free(b);
+  std::lock_guard<std::mutex> lock(bin_mutex());
+  BinCache &c = bin_cache();
+  if (int rc = bin_cache_reserve(c, nrows * ncols, np)) {
+    bin_cache_drop(c);
     return rc;
   }
-  int *slot = nullptr;
-  if (np > 0 && hipMalloc(&slot, sizeof(int) * np) != hipSuccess) {
-    binning_free(b);
-    return fail(AMSC_ERR_NOMEM, "amsc_bin_particles: slot allocation");
-  }
-  int rc = launch_key_count(np, d_x, d_y, nrows, ncols, hx, hy, d_p2g, slot,
-                            b.count, nullptr, st);
-  if (!rc) rc = launch_finish_binning(b, np, d_p2g, slot, d_cptcls, nullptr, st);
-  if (!rc) {
-    hipError_t e = hipMemcpyAsync(d_cstart, b.cstart,
-                                  sizeof(int) * ((size_t)nrows * ncols + 1),
-                                  hipMemcpyDeviceToDevice, st);
-    if (e != hipSuccess) rc = fail(AMSC_ERR_HIP, hipGetErrorString(e));
-  }
+  Binning b;
+  b.nrows = nrows;
+  b.ncols = ncols;
+  b.count = c.count;
+  b.cstart = d_cstart;  // the scan writes the caller's array directly
+  b.scan = c.scan;
+  b.scan.owned = false;
+  int rc = launch_key_count(np, d_x, d_y, nrows, ncols, hx, hy, d_p2g, c.slot,
+                            c.count, nullptr, st);
+  if (!rc) rc = launch_finish_binning(b, np, d_p2g, c.slot, d_cptcls, nullptr, st);
   if (!rc && coloff) rc = launch_colouring(nrows, ncols, d_ccidx, coloff, st);
-  // the temporaries are released after the stream has used them
+  // complete before returning, like the host rebuild it replaces
   hipError_t e = hipStreamSynchronize(st);
   if (!rc && e != hipSuccess) rc = fail(AMSC_ERR_HIP, hipGetErrorString(e));
-  if (slot) (void)hipFree(slot);
-  binning_free(b);
+  if (rc) bin_cache_drop(c);  // the counts may not be back at zero
   return rc;
 }

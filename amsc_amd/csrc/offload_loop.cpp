@@ -14,6 +14,7 @@
 //          float64 particle arrays (np each, kParticle order), Z (nn)
 // out.bin: float64 particle arrays (kParticle order), nodal arrays
 //          (kNodal order), then int32 p2g (np), cstart (nc+1), cptcls (np)
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -92,6 +93,8 @@ int main(int argc, char **argv) {
      "binning");
 
   namespace gk = gpu_kernels;
+  ok(amsc_synchronize(), "synchronize");
+  const auto t0 = std::chrono::steady_clock::now();
   for (int it = 0; it < nsteps; ++it) {
     // K1, gpu_offload.cpp:410-422
     gk::reset_nodal_arrays(nn, v[MV], v[MOMVX], v[MOMVY], v[FEXTX], v[FEXTY],
@@ -154,6 +157,9 @@ int main(int argc, char **argv) {
        "binning");
   }
 
+  ok(amsc_synchronize(), "synchronize");
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   FILE *out = std::fopen(argv[2], "wb");
   if (!out) die("cannot open the output file");
   for (int k = 0; k < NP_FIELDS; ++k) {
@@ -181,6 +187,8 @@ int main(int argc, char **argv) {
   amsc_device_free(cptcls);
   amsc_device_free(ccidx);
   if (bad) die("write failed");
-  std::printf("amsc_offload_loop: np=%d steps=%d\n", np, nsteps);
+  // the reference driver's PERF line (gpu_offload.cpp:580-586)
+  std::printf("PERF np=%d counted_steps=%d tot_s=%.6f per_step_ms=%.4f\n", np,
+              nsteps, secs, nsteps > 0 ? 1e3 * secs / nsteps : 0.0);
   return 0;
 }
