@@ -114,8 +114,29 @@ struct Recip {
 };
 inline Recip make_recip(double hx, double hy) {
   int ex = 0, ey = 0;
-  const bool p2 = std::frexp(hx, &ex) == 0.5 && std::frexp(hy, &ey) == 0.5;
+  const bool p2 = std::fpclassify(hx) == FP_NORMAL &&
+                  std::fpclassify(hy) == FP_NORMAL &&
+                  std::frexp(hx, &ex) == 0.5 && std::frexp(hy, &ey) == 0.5;
   return Recip{1.0 / hx, -1.0 / hx, 1.0 / hy, -1.0 / hy, p2 ? 1 : 0};
+}
+
+// h is a positive normal power of two (its mantissa bits are zero).
+__device__ __forceinline__ bool pow2_dev(double h) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(h);
+  const unsigned e = (unsigned)(b >> 52) & 0x7FFu;
+  return (b & 0x000FFFFFFFFFFFFFull) == 0 && e != 0 && e != 0x7FFu &&
+         (b >> 63) == 0;
+}
+
+// make_recip on the device (kernels given hx, hy as arguments).
+__device__ __forceinline__ Recip recip_dev(double hx, double hy) {
+  Recip R;
+  R.ix = 1.0 / hx;
+  R.nix = -1.0 / hx;
+  R.iy = 1.0 / hy;
+  R.niy = -1.0 / hy;
+  R.pow2 = pow2_dev(hx) && pow2_dev(hy) ? 1 : 0;
+  return R;
 }
 
 // local_coords with the grid's reciprocals: the same values.

@@ -80,12 +80,22 @@ int prepare_colouring(int nrows, const int *ccidx, const int *coloff,
 }
 
 // The (up to 4) cells around node (c, r), sorted by colour; corner is the
-// node's local index in that cell.
+// node's local index in that cell.  Entries [0, n) are the cells that exist.
 struct NodeCells {
   int cell[4];
   int cc[4], cr[4], corner[4];
   int n;
 };
+
+__device__ __forceinline__ void nc_swap(NodeCells &a, int *key, int i, int j) {
+  if (key[j] < key[i]) {
+    int t = key[i]; key[i] = key[j]; key[j] = t;
+    t = a.cell[i]; a.cell[i] = a.cell[j]; a.cell[j] = t;
+    t = a.cc[i]; a.cc[i] = a.cc[j]; a.cc[j] = t;
+    t = a.cr[i]; a.cr[i] = a.cr[j]; a.cr[j] = t;
+    t = a.corner[i]; a.corner[i] = a.corner[j]; a.corner[j] = t;
+  }
+}
 
 __device__ __forceinline__ NodeCells node_cells(int c, int r, int nrows,
                                                 int ncols,
@@ -93,31 +103,28 @@ __device__ __forceinline__ NodeCells node_cells(int c, int r, int nrows,
   NodeCells nc;
   nc.n = 0;
   int key[4];
-  // (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r) corner 0
+  // (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r) corner 0;
+  // key = colour * 4 + k (a missing cell sorts last); a 4-key sorting
+  // network keeps every index a constant (registers, no scratch)
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int ccol = c - 1 + (k >> 1);
     const int crow = r - 1 + (k & 1);
-    if (ccol < 0 || ccol >= ncols || crow < 0 || crow >= nrows) continue;
-    const int cell = crow + ccol * nrows;
-    const int col = colour[cell];
-    if (col == 0xFF) continue;
-    int i = nc.n++;
-    // insertion by (colour, cell)
-    while (i > 0 && (key[i - 1] > col * 4 + k)) {
-      key[i] = key[i - 1];
-      nc.cell[i] = nc.cell[i - 1];
-      nc.cc[i] = nc.cc[i - 1];
-      nc.cr[i] = nc.cr[i - 1];
-      nc.corner[i] = nc.corner[i - 1];
-      --i;
-    }
-    key[i] = col * 4 + k;
-    nc.cell[i] = cell;
-    nc.cc[i] = ccol;
-    nc.cr[i] = crow;
-    nc.corner[i] = 3 - k;
+    const bool in = ccol >= 0 && ccol < ncols && crow >= 0 && crow < nrows;
+    const int cell = in ? crow + ccol * nrows : 0;
+    const int col = in ? colour[cell] : 0xFF;
+    key[k] = col != 0xFF ? col * 4 + k : 64 + k;
+    nc.n += col != 0xFF;
+    nc.cell[k] = cell;
+    nc.cc[k] = ccol;
+    nc.cr[k] = crow;
+    nc.corner[k] = 3 - k;
   }
+  nc_swap(nc, key, 0, 1);
+  nc_swap(nc, key, 2, 3);
+  nc_swap(nc, key, 0, 2);
+  nc_swap(nc, key, 1, 3);
+  nc_swap(nc, key, 1, 2);
   return nc;
 }
 
@@ -145,12 +152,15 @@ __global__ void __launch_bounds__(kBlock)
   double acc[NF];
 #pragma unroll
   for (int k = 0; k < NF; ++k) acc[k] = f.out[k][n];
-  for (int q = 0; q < cells.n; ++q) {
+  const Recip R = recip_dev(hx, hy);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q >= cells.n) break;
     const int cell = cells.cell[q];
     const int e = cstart[cell + 1];
     for (int jp = cstart[cell]; jp < e; ++jp) {
       const int ip = cptcls[jp];
-      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy);
+      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy, R);
       const double N = shape(L, cells.corner[q]);
 #pragma unroll
       for (int k = 0; k < NF; ++k) acc[k] += N * f.in[k][ip];
@@ -182,12 +192,15 @@ __global__ void __launch_bounds__(kBlock)
   const NodeCells cells = node_cells(c, r, nrows, ncols, colour);
   if (cells.n == 0) return;
   double ax = Fix[n], ay = Fiy[n];
-  for (int q = 0; q < cells.n; ++q) {
+  const Recip R = recip_dev(hx, hy);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q >= cells.n) break;
     const int cell = cells.cell[q];
     const int e = cstart[cell + 1];
     for (int jp = cstart[cell]; jp < e; ++jp) {
       const int ip = cptcls[jp];
-      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy);
+      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy, R);
       const double Nx = shape_dx(L, cells.corner[q], hx);
       const double Ny = shape_dy(L, cells.corner[q], hy);
       const double vp = Vp[ip];
@@ -236,7 +249,7 @@ __global__ void gradZ_kernel(int np, int nrows, double hx, double hy,
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np) return;
   const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
-  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy, recip_dev(hx, hy));
   double dzx = 0.0, dzy = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -344,7 +357,7 @@ __global__ void g2p_advect_kernel(int np, int nrows, int ncols, double hx,
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np) return;
   const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
-  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy, recip_dev(hx, hy));
   double ax = 0, ay = 0, vxL = 0, vyL = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -382,7 +395,7 @@ __global__ void g2pd_height_kernel(int np, int nrows, double hx, double hy,
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np) return;
   const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
-  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy, recip_dev(hx, hy));
   double vxdx = 0, vxdy = 0, vydx = 0, vydy = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -473,7 +486,7 @@ __global__ void g2p_height_kernel(int np, int nrows, double hx, double hy,
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np) return;
   const int ci = p2g[ip], r = ci % nrows, c = ci / nrows;
-  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy);
+  const Local L = local_coords(x[ip], y[ip], c, r, hx, hy, recip_dev(hx, hy));
   double zp = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) zp += shape(L, k) * Z[node_id(k, c, r, nrows)];
