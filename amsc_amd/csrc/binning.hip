@@ -134,35 +134,6 @@ __device__ __forceinline__ void load4(const int *in, long long base, int n,
   }
 }
 
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  const int lane = __lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
-// Exclusive scan of one value per thread over the workgroup (blockDim.x a
-// multiple of 64, at most 1024); *total = the workgroup's sum.  s_w holds 17
-// ints; the caller synchronises before reusing it.
-__device__ __forceinline__ int block_excl_scan(int v, int *s_w, int *total) {
-  const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int inc = wave_incl_scan(v);
-  if (lane == 63) s_w[w] = inc;
-  __syncthreads();
-  if (w == 0) {
-    const int t = lane < nw ? s_w[lane] : 0;
-    const int ti = wave_incl_scan(t);
-    if (lane < nw) s_w[lane] = ti - t;
-    if (lane == nw - 1) s_w[16] = ti;
-  }
-  __syncthreads();
-  *total = s_w[16];
-  return s_w[w] + inc - v;
-}
-
 __global__ void __launch_bounds__(kScanThreads)
     scan_sums_kernel(const int *__restrict__ in, int n, int *tile_sum) {
   __shared__ int s_w[17];

@@ -3,15 +3,16 @@
 //
 // Per-particle and per-node entry points are one thread per item with the
 // reference's expressions.  The four colour-scattered P2G transfers
-// (gpu_kernels.cpp:97-128,183-246,263-296,572-609) become one node-centric
-// gather each: a node reads the particles of its (up to 4) cells in colour
-// order, each cell's particles in cptcls order, and adds them to its current
-// value.  That is the exact addition sequence the colour loop produces (cells
-// of one colour never share a node), without atomics, without 4 launches and
+// (gpu_kernels.cpp:97-128,183-246,263-296,572-609) become one LDS-tiled
+// gather each: a node adds the particles of its (up to 4) cells in colour
+// order, each cell's particles in cptcls order, to its current value.  That
+// is the exact addition sequence the colour loop produces (cells of one
+// colour never share a node), without atomics, without 4 launches and
 // without read-modify-write traffic on the nodal arrays.
 #include "common.hpp"
 #include "amsc_mpm.h"
 
+#include <climits>
 #include <mutex>
 
 namespace amsc {
@@ -79,151 +80,180 @@ int prepare_colouring(int nrows, const int *ccidx, const int *coloff,
   return AMSC_OK;
 }
 
-// The (up to 4) cells around node (c, r), sorted by colour; corner is the
-// node's local index in that cell.  Entries [0, n) are the cells that exist.
-struct NodeCells {
-  int cell[4];
-  int cc[4], cr[4], corner[4];
-  int n;
+// ---- the colour-ordered scatters as LDS-tiled gathers.
+// A workgroup owns a tile of kTC x kTR nodes (one thread each) and the
+// (kTC+1) x (kTR+1) cells around it.  The tile's particles are listed cell by
+// cell in colour-major order (all its colour-0 cells, then colour 1, ...;
+// box order within a colour) and staged through LDS 256 at a time; each node
+// adds the particles of its cells in list order, i.e. its cells in colour
+// order and each cell's particles in cptcls order, starting from the node's
+// current value: the addition sequence of the reference's colour loop.
+constexpr int kTC = 8, kTR = 32, kTNT = kTC * kTR;
+constexpr int kTCells = (kTC + 1) * (kTR + 1);
+constexpr int kTCH = kTNT, kTCHP = kTCH + kTCH / 16;
+static_assert(kTCells <= 2 * kTNT, "two box cells per thread");
+
+// Staged position i at i + i/16 (node threads walk neighbouring cells at a
+// stride of a few positions).
+__device__ __forceinline__ int tg_slot(int i) { return i + (i >> 4); }
+
+template <int NIN, int NOUT>
+struct TGFields {
+  const double *in[NIN];
+  double *out[NOUT];
 };
 
-__device__ __forceinline__ void nc_swap(NodeCells &a, int *key, int i, int j) {
-  if (key[j] < key[i]) {
-    int t = key[i]; key[i] = key[j]; key[j] = t;
-    t = a.cell[i]; a.cell[i] = a.cell[j]; a.cell[j] = t;
-    t = a.cc[i]; a.cc[i] = a.cc[j]; a.cc[j] = t;
-    t = a.cr[i]; a.cr[i] = a.cr[j]; a.cr[j] = t;
-    t = a.corner[i]; a.corner[i] = a.corner[j]; a.corner[j] = t;
-  }
-}
-
-__device__ __forceinline__ NodeCells node_cells(int c, int r, int nrows,
-                                                int ncols,
-                                                const uint8_t *colour) {
-  NodeCells nc;
-  nc.n = 0;
-  int key[4];
-  // (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r) corner 0;
-  // key = colour * 4 + k (a missing cell sorts last); a 4-key sorting
-  // network keeps every index a constant (registers, no scratch)
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int ccol = c - 1 + (k >> 1);
-    const int crow = r - 1 + (k & 1);
-    const bool in = ccol >= 0 && ccol < ncols && crow >= 0 && crow < nrows;
-    const int cell = in ? crow + ccol * nrows : 0;
-    const int col = in ? colour[cell] : 0xFF;
-    key[k] = col != 0xFF ? col * 4 + k : 64 + k;
-    nc.n += col != 0xFF;
-    nc.cell[k] = cell;
-    nc.cc[k] = ccol;
-    nc.cr[k] = crow;
-    nc.corner[k] = 3 - k;
-  }
-  nc_swap(nc, key, 0, 1);
-  nc_swap(nc, key, 2, 3);
-  nc_swap(nc, key, 0, 2);
-  nc_swap(nc, key, 1, 3);
-  nc_swap(nc, key, 1, 2);
-  return nc;
-}
-
-template <int NF>
-struct Fields {
-  const double *in[NF];
-  double *out[NF];
-};
-
-// out[f][n] += sum N_corner(x_p) * in[f][p], reference order.
-template <int NF>
-__global__ void __launch_bounds__(kBlock)
-    gather_shape_kernel(int nrows, int ncols, double hx, double hy,
+// INTERNAL: in = {Vp, F11, F12, F21, F22}, out = {F_int_vx, F_int_vy},
+// out += (Nx*F11 + Ny*F21)*Vp, (Nx*F12 + Ny*F22)*Vp (gpu_kernels.cpp:286-292);
+// otherwise out[k] += N * in[k].
+template <int NIN, int NOUT, bool INTERNAL>
+__global__ void __launch_bounds__(kTNT)
+    tiled_gather_kernel(int nrows, int ncols, double hx, double hy, int ntx,
                         const int *__restrict__ cstart,
                         const int *__restrict__ cptcls,
                         const uint8_t *__restrict__ colour,
                         const double *__restrict__ x,
-                        const double *__restrict__ y, Fields<NF> f) {
-  const int nn = (nrows + 1) * (ncols + 1);
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
-  const int c = n / (nrows + 1), r = n % (nrows + 1);
-  const NodeCells cells = node_cells(c, r, nrows, ncols, colour);
-  if (cells.n == 0) return;
-  double acc[NF];
-#pragma unroll
-  for (int k = 0; k < NF; ++k) acc[k] = f.out[k][n];
+                        const double *__restrict__ y, TGFields<NIN, NOUT> f) {
+  __shared__ int s_beg[kTCells], s_cnt[kTCells], s_off[kTCells];
+  __shared__ unsigned short s_own[kTCH];
+  __shared__ double s_xy[2][kTCHP];
+  __shared__ double s_in[NIN][kTCHP];
+  __shared__ int s_w[17];
+  const int t = threadIdx.x;
+  const int c0 = (blockIdx.x % ntx) * kTC, r0 = (blockIdx.x / ntx) * kTR;
+  const int ccl = c0 - 1 < 0 ? 0 : c0 - 1;
+  const int cch = c0 + kTC - 1 > ncols - 1 ? ncols - 1 : c0 + kTC - 1;
+  const int rl = r0 - 1 < 0 ? 0 : r0 - 1;
+  const int rh = r0 + kTR - 1 > nrows - 1 ? nrows - 1 : r0 + kTR - 1;
+  const int nr = rh - rl + 1, ncl = (cch - ccl + 1) * nr;
   const Recip R = recip_dev(hx, hy);
+
+  // cell table: thread t holds box cells 2t and 2t+1 (column-major order)
+  int cnt[2] = {0, 0}, col[2] = {0xFF, 0xFF};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (q >= cells.n) break;
-    const int cell = cells.cell[q];
-    const int e = cstart[cell + 1];
-    for (int jp = cstart[cell]; jp < e; ++jp) {
-      const int ip = cptcls[jp];
-      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy, R);
-      const double N = shape(L, cells.corner[q]);
-#pragma unroll
-      for (int k = 0; k < NF; ++k) acc[k] += N * f.in[k][ip];
+  for (int u = 0; u < 2; ++u) {
+    const int j = 2 * t + u;
+    if (j < ncl) {
+      const int cell = (ccl + j / nr) * nrows + (rl + j % nr);
+      const int b = cstart[cell];
+      col[u] = colour[cell];
+      cnt[u] = col[u] == 0xFF ? 0 : cstart[cell + 1] - b;
+      s_beg[j] = b;
+      s_cnt[j] = cnt[u];
     }
   }
-#pragma unroll
-  for (int k = 0; k < NF; ++k) f.out[k][n] = acc[k];
-}
-
-// F_int_vx += (Nx*F11 + Ny*F21)*Vp, F_int_vy += (Nx*F12 + Ny*F22)*Vp
-// (gpu_kernels.cpp:286-292)
-__global__ void __launch_bounds__(kBlock)
-    gather_internal_kernel(int nrows, int ncols, double hx, double hy,
-                           const int *__restrict__ cstart,
-                           const int *__restrict__ cptcls,
-                           const uint8_t *__restrict__ colour,
-                           const double *__restrict__ x,
-                           const double *__restrict__ y,
-                           const double *__restrict__ Vp,
-                           const double *__restrict__ F11,
-                           const double *__restrict__ F12,
-                           const double *__restrict__ F21,
-                           const double *__restrict__ F22, double *Fix,
-                           double *Fiy) {
-  const int nn = (nrows + 1) * (ncols + 1);
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
-  const int c = n / (nrows + 1), r = n % (nrows + 1);
-  const NodeCells cells = node_cells(c, r, nrows, ncols, colour);
-  if (cells.n == 0) return;
-  double ax = Fix[n], ay = Fiy[n];
-  const Recip R = recip_dev(hx, hy);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (q >= cells.n) break;
-    const int cell = cells.cell[q];
-    const int e = cstart[cell + 1];
-    for (int jp = cstart[cell]; jp < e; ++jp) {
-      const int ip = cptcls[jp];
-      const Local L = local_coords(x[ip], y[ip], cells.cc[q], cells.cr[q], hx, hy, R);
-      const double Nx = shape_dx(L, cells.corner[q], hx);
-      const double Ny = shape_dy(L, cells.corner[q], hy);
-      const double vp = Vp[ip];
-      ax += (Nx * F11[ip] + Ny * F21[ip]) * vp;
-      ay += (Nx * F12[ip] + Ny * F22[ip]) * vp;
-    }
+  // tile-list offsets, colour-major
+  int base = 0, off[2] = {0, 0};
+  for (int k = 0; k < 4; ++k) {
+    const int v0 = col[0] == k ? cnt[0] : 0, v1 = col[1] == k ? cnt[1] : 0;
+    int total;
+    const int ex = block_excl_scan(v0 + v1, s_w, &total);
+    if (col[0] == k) off[0] = base + ex;
+    if (col[1] == k) off[1] = base + ex + v0;
+    base += total;
+    __syncthreads();
   }
-  Fix[n] = ax;
-  Fiy[n] = ay;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    if (2 * t + u < ncl) s_off[2 * t + u] = off[u];
+  const int L = base;
+  __syncthreads();
+
+  // this node's cells as tile-list ranges, in list order
+  const int c = c0 + t / kTR, r = r0 + t % kTR;
+  const bool node_ok = c <= ncols && r <= nrows;
+  const int n = r + c * (nrows + 1);
+  int tb[4], te[4], cn[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    // (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r) corner 0
+    const int cc = c - 1 + (k >> 1), cr = r - 1 + (k & 1);
+    const bool in = node_ok && cc >= 0 && cc < ncols && cr >= 0 && cr < nrows;
+    const int j = in ? (cc - ccl) * nr + (cr - rl) : 0;
+    tb[k] = in ? s_off[j] : INT_MAX;
+    te[k] = in ? s_off[j] + s_cnt[j] : INT_MAX;
+    cn[k] = 3 - k;
+  }
+  auto cx = [&](int i, int j) {
+    if (tb[j] < tb[i]) {
+      int q = tb[i]; tb[i] = tb[j]; tb[j] = q;
+      q = te[i]; te[i] = te[j]; te[j] = q;
+      q = cn[i]; cn[i] = cn[j]; cn[j] = q;
+    }
+  };
+  cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
+  double acc[NOUT];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) acc[o] = node_ok ? f.out[o][n] : 0.0;
+
+  for (int b0 = 0; b0 < L; b0 += kTCH) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = 2 * t + u;
+      if (j < ncl) {
+        const int lo0 = s_off[j] > b0 ? s_off[j] : b0;
+        const int hi0 = s_off[j] + cnt[u] < b0 + kTCH ? s_off[j] + cnt[u] : b0 + kTCH;
+        for (int q = lo0; q < hi0; ++q) s_own[q - b0] = (unsigned short)j;
+      }
+    }
+    __syncthreads();
+    const int q = b0 + t;
+    if (q < L) {
+      const int j = s_own[t];
+      const int ip = cptcls[s_beg[j] + (q - s_off[j])];
+      const Local Lc = local_coords(x[ip], y[ip], ccl + j / nr, rl + j % nr,
+                                    hx, hy, R);
+      const int i = tg_slot(t);
+      s_xy[0][i] = Lc.xi;
+      s_xy[1][i] = Lc.eta;
+#pragma unroll
+      for (int m = 0; m < NIN; ++m) s_in[m][i] = f.in[m][ip];
+    }
+    __syncthreads();
+    if (node_ok) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int lo = tb[k] > b0 ? tb[k] : b0;
+        const int hi = te[k] < b0 + kTCH ? te[k] : b0 + kTCH;
+        for (int qq = lo; qq < hi; ++qq) {
+          const int i = tg_slot(qq - b0);
+          Local Lq;
+          Lq.xi = s_xy[0][i];
+          Lq.eta = s_xy[1][i];
+          if (INTERNAL) {
+            const double Nx = shape_dx(Lq, cn[k], hx);
+            const double Ny = shape_dy(Lq, cn[k], hy);
+            const double vp = s_in[0][i];
+            acc[0] += (Nx * s_in[1][i] + Ny * s_in[3][i]) * vp;
+            acc[NOUT - 1] += (Nx * s_in[2][i] + Ny * s_in[4][i]) * vp;
+          } else {
+            const double N = shape(Lq, cn[k]);
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o) acc[o] += N * s_in[o][i];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (node_ok)
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) f.out[o][n] = acc[o];
 }
 
-template <int NF>
-int gather_shape(int nrows, double hx, double hy, const int *cstart,
+template <int NIN, int NOUT, bool INTERNAL>
+int tiled_gather(int nrows, double hx, double hy, const int *cstart,
                  const int *cptcls, const int *ccidx, const int *coloff,
-                 const double *x, const double *y, Fields<NF> f) {
+                 const double *x, const double *y, TGFields<NIN, NOUT> f) {
   int ncols = 0;
   const uint8_t *colour = nullptr;
-  int rc = prepare_colouring(nrows, ccidx, coloff, &ncols, &colour);
-  if (rc) return rc;
+  if (int rc = prepare_colouring(nrows, ccidx, coloff, &ncols, &colour))
+    return rc;
   if (ncols == 0) return AMSC_OK;
-  const int nn = (nrows + 1) * (ncols + 1);
-  gather_shape_kernel<NF><<<grid_for(nn, kBlock), kBlock, 0, current_stream()>>>(
-      nrows, ncols, hx, hy, cstart, cptcls, colour, x, y, f);
+  const int ntx = (ncols + 1 + kTC - 1) / kTC, nty = (nrows + 1 + kTR - 1) / kTR;
+  tiled_gather_kernel<NIN, NOUT, INTERNAL>
+      <<<ntx * nty, kTNT, 0, current_stream()>>>(nrows, ncols, hx, hy, ntx,
+                                                 cstart, cptcls, colour, x, y, f);
   AMSC_LAUNCH_CHECK();
   return AMSC_OK;
 }
@@ -558,8 +588,8 @@ int amsc_scatter_mass_momentum(int np, int nrows, double hx, double hy,
                                const double *d_mom_py, double *d_Mv,
                                double *d_mom_vx, double *d_mom_vy) {
   (void)np;
-  Fields<3> f = {{d_Mp, d_mom_px, d_mom_py}, {d_Mv, d_mom_vx, d_mom_vy}};
-  return gather_shape<3>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+  TGFields<3, 3> f = {{d_Mp, d_mom_px, d_mom_py}, {d_Mv, d_mom_vx, d_mom_vy}};
+  return tiled_gather<3, 3, false>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
                          d_y, f);
 }
 
@@ -599,8 +629,8 @@ int amsc_scatter_gravity_force(int np, int nrows, double hx, double hy,
                                const double *d_Fpx, const double *d_Fpy,
                                double *d_FPxv, double *d_FPyv) {
   (void)np;
-  Fields<2> f = {{d_Fpx, d_Fpy}, {d_FPxv, d_FPyv}};
-  return gather_shape<2>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+  TGFields<2, 2> f = {{d_Fpx, d_Fpy}, {d_FPxv, d_FPyv}};
+  return tiled_gather<2, 2, false>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
                          d_y, f);
 }
 
@@ -612,8 +642,8 @@ int amsc_scatter_friction_force(int np, int nrows, double hx, double hy,
                                 const double *d_Fric_py, double *d_Fric_x,
                                 double *d_Fric_y) {
   (void)np;
-  Fields<2> f = {{d_Fric_px, d_Fric_py}, {d_Fric_x, d_Fric_y}};
-  return gather_shape<2>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+  TGFields<2, 2> f = {{d_Fric_px, d_Fric_py}, {d_Fric_x, d_Fric_y}};
+  return tiled_gather<2, 2, false>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
                          d_y, f);
 }
 
@@ -635,16 +665,9 @@ int amsc_scatter_internal_forces(int np, int nrows, double hx, double hy,
                                  const double *d_F22, double *d_F_int_vx,
                                  double *d_F_int_vy) {
   (void)np;
-  int ncols = 0;
-  const uint8_t *colour = nullptr;
-  if (int rc = prepare_colouring(nrows, ccidx, coloff, &ncols, &colour))
-    return rc;
-  if (ncols == 0) return AMSC_OK;
-  const int nn = (nrows + 1) * (ncols + 1);
-  LAUNCH1D(gather_internal_kernel, nn, nrows, ncols, hx, hy, cstart, cptcls,
-           colour, d_x, d_y, d_Vp, d_F11, d_F12, d_F21, d_F22, d_F_int_vx,
-           d_F_int_vy);
-  return AMSC_OK;
+  TGFields<5, 2> f = {{d_Vp, d_F11, d_F12, d_F21, d_F22}, {d_F_int_vx, d_F_int_vy}};
+  return tiled_gather<5, 2, true>(nrows, hx, hy, cstart, cptcls, ccidx, coloff,
+                                  d_x, d_y, f);
 }
 
 int amsc_nodal_force_and_momentum(int nn, double dt, const double *d_F_ext_vx,
@@ -757,9 +780,9 @@ int amsc_scatter_plotvars(int np, int nrows, double hx, double hy,
                           double *d_pv_vvy, double *d_pv_avx,
                           double *d_pv_avy, double *d_HV) {
   (void)np;
-  Fields<6> f = {{d_Mp, d_vpx, d_vpy, d_apx, d_apy, d_hp},
+  TGFields<6, 6> f = {{d_Mp, d_vpx, d_vpy, d_apx, d_apy, d_hp},
                  {d_pv_rho, d_pv_vvx, d_pv_vvy, d_pv_avx, d_pv_avy, d_HV}};
-  return gather_shape<6>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
+  return tiled_gather<6, 6, false>(nrows, hx, hy, cstart, cptcls, ccidx, coloff, d_x,
                          d_y, f);
 }
 
