@@ -120,6 +120,10 @@ def lib():
                                      _ip]
     L.amsc_ms_decide.argtypes = [C.POINTER(AmscParams), C.c_int, C.c_double] + \
         [_vp] * 15 + [_ip]
+    L.amsc_ms_execute.argtypes = [C.POINTER(AmscParams), C.c_int, _vp, _vp, C.c_int,
+                                  C.POINTER(C.c_char_p), C.POINTER(_vp),
+                                  C.POINTER(_vp), _vp, _vp, _vp, _vp, C.c_int,
+                                  C.POINTER(C.c_int)]
     L.amsc_sim_create.argtypes = [C.POINTER(AmscParams), C.c_int] + [_dp] * 11 + \
         [C.c_int, C.POINTER(_vp)]
     L.amsc_sim_create_from_json.argtypes = [C.c_char_p, C.c_int, C.c_int,
@@ -207,6 +211,22 @@ def ms_decide(prm: dict, np_, dt, ptrs, d_level, d_cstart, d_cptcls,
     check(lib().amsc_ms_decide(C.byref(P), np_, dt, *ptrs, d_level, d_cstart,
                                d_cptcls, d_action, d_partner, counts))
     return counts
+
+
+def ms_execute(prm: dict, np_, d_action, d_partner, names, d_in, d_out,
+               d_label_in, d_level_in, d_label_out, d_level_out, capacity):
+    """amsc_ms_execute; names/d_in/d_out: lists (field names, device
+    pointers).  Returns the new particle count."""
+    nf = len(names)
+    cn = (C.c_char_p * nf)(*[n.encode() for n in names])
+    pin = (_vp * nf)(*d_in)
+    pout = (_vp * nf)(*d_out)
+    nnew = C.c_int(0)
+    P = AmscParams.from_dict(prm)
+    check(lib().amsc_ms_execute(C.byref(P), np_, d_action, d_partner, nf, cn, pin,
+                                pout, d_label_in, d_level_in, d_label_out,
+                                d_level_out, capacity, C.byref(nnew)))
+    return nnew.value
 
 
 class DeviceArray:

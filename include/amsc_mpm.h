@@ -241,6 +241,26 @@ int amsc_ms_decide(const amsc_params *p, int np, double dt, const double *d_x,
                    const int *d_cptcls, int *d_action, int *d_partner,
                    int *counts);
 
+/* execute_merge_split (merge_split_ops_cmes.h:258-427) on the caller's
+ * device arrays, after amsc_ms_decide: replaces the host
+ * adaptive_merge_split of gpu_offload.cpp:321-401 for a caller that keeps
+ * its own particle storage.  nfields particle fields (at most 64) named as
+ * the reference's dprops: "x" "y" "Mp" "Vp" "Ap" "hp" "vpx" "vpy" "vpx_dx"
+ * "vpx_dy" "vpy_dx" "vpy_dy" are required, "mom_px" "mom_py" "xp" "yp" are
+ * recomputed when present, any other field is carried (mass-averaged on a
+ * merge, copied on a split; Mp Vp Ap mom_px mom_py are extensive).  d_in[k]
+ * holds np values, d_out[k] room for `capacity` values (out of place);
+ * labels and levels likewise.  *np_out = the new particle count, in the
+ * reference's order (kept, split and merged particles at their scan
+ * offsets).  AMSC_ERR_CAPACITY if it would exceed capacity (nothing
+ * written).  Synchronous. */
+int amsc_ms_execute(const amsc_params *p, int np, const int *d_action,
+                    const int *d_partner, int nfields,
+                    const char *const *names, const double *const *d_in,
+                    double *const *d_out, const int *d_label_in,
+                    const int *d_level_in, int *d_label_out,
+                    int *d_level_out, int capacity, int *np_out);
+
 /* ------------------------------------------------------------------ 3. */
 typedef struct amsc_sim amsc_sim;
 

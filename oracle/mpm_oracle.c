@@ -936,8 +936,13 @@ void or_ms_decide(const or_params *p, int np, double dt, const double *x,
 
 /* execute_merge_split, merge_split_ops_cmes.h:258-427: exclusive scan of
  * weights {1,2,1,0}, then new SoA in input order. */
-static void ms_execute(or_sim *s, const int *act, const int *partner) {
-  const int N = s->np;
+/* execute_merge_split (merge_split_ops_cmes.h:258-427) from arrays: f and
+ * nf hold the OR_NFIELDS fields in FIELD_NAMES order, nf/nlabel/nlevel room
+ * for the new count (np + splits); returns the new count. */
+static int ms_execute_core(const or_params *p, int N, const int *act,
+                           const int *partner, double *const *f,
+                           const int *label, const int *level, double **nf,
+                           int *nlabel, int *nlevel) {
   static const int weights[4] = {1, 2, 1, 0};
   int *offset = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
   int Nnew = 0;
@@ -945,23 +950,17 @@ static void ms_execute(or_sim *s, const int *act, const int *partner) {
     offset[i] = Nnew;
     Nnew += weights[act[i]];
   }
-  double *nf[OR_NFIELDS];
-  for (int k = 0; k < OR_NFIELDS; ++k)
-    nf[k] = (double *)malloc(sizeof(double) * (Nnew > 0 ? Nnew : 1));
-  int *nlabel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
-  int *nlevel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
-  const double hx = s->p.hx, hy = s->p.hy;
-  const double Lx = s->p.ncols * hx, Ly = s->p.nrows * hy, eps = 1e-10;
+  const double hx = p->hx, hy = p->hy;
+  const double Lx = p->ncols * hx, Ly = p->nrows * hy, eps = 1e-10;
 #define CLAMPX(v) ((v) < eps ? eps : ((v) > Lx - eps ? Lx - eps : (v)))
 #define CLAMPY(v) ((v) < eps ? eps : ((v) > Ly - eps ? Ly - eps : (v)))
-  double **f = s->f;
   for (int i = 0; i < N; ++i) {
     const int a = act[i];
     if (a == 0) {
       const int o = offset[i];
       for (int k = 0; k < OR_NFIELDS; ++k) nf[k][o] = f[k][i];
-      nlabel[o] = s->label[i];
-      nlevel[o] = s->level[i];
+      nlabel[o] = label[i];
+      nlevel[o] = level[i];
     } else if (a == 1) {
       const int o1 = offset[i], o2 = offset[i] + 1;
       double ux, uy;
@@ -990,7 +989,7 @@ static void ms_execute(or_sim *s, const int *act, const int *partner) {
       nf[OR_MOM_PX][o2] = nf[OR_MP][o2] * nf[OR_VPX][o2];
       nf[OR_MOM_PY][o1] = nf[OR_MP][o1] * nf[OR_VPY][o1];
       nf[OR_MOM_PY][o2] = nf[OR_MP][o2] * nf[OR_VPY][o2];
-      nlevel[o1] = nlevel[o2] = s->level[i] - 1;
+      nlevel[o1] = nlevel[o2] = level[i] - 1;
       nlabel[o1] = nlabel[o2] = -1;
     } else if (a == 2) {
       const int j = partner[i], o = offset[i];
@@ -1007,12 +1006,32 @@ static void ms_execute(or_sim *s, const int *act, const int *partner) {
       nf[OR_XP][o] = nx; nf[OR_YP][o] = ny;
       nf[OR_MOM_PX][o] = nf[OR_MP][o] * nf[OR_VPX][o];
       nf[OR_MOM_PY][o] = nf[OR_MP][o] * nf[OR_VPY][o];
-      nlevel[o] = s->level[i] + 1;
+      nlevel[o] = level[i] + 1;
       nlabel[o] = -1;
     }
   }
 #undef CLAMPX
 #undef CLAMPY
+  free(offset);
+  return Nnew;
+}
+
+static int ms_new_count(int N, const int *act) {
+  static const int weights[4] = {1, 2, 1, 0};
+  int Nnew = 0;
+  for (int i = 0; i < N; ++i) Nnew += weights[act[i]];
+  return Nnew;
+}
+
+static void ms_execute(or_sim *s, const int *act, const int *partner) {
+  const int N = s->np, Nnew = ms_new_count(N, act);
+  double *nf[OR_NFIELDS];
+  for (int k = 0; k < OR_NFIELDS; ++k)
+    nf[k] = (double *)malloc(sizeof(double) * (Nnew > 0 ? Nnew : 1));
+  int *nlabel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
+  int *nlevel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
+  ms_execute_core(&s->p, N, act, partner, s->f, s->label, s->level, nf,
+                  nlabel, nlevel);
   sim_reserve(s, Nnew > 0 ? Nnew : 1);
   for (int k = 0; k < OR_NFIELDS; ++k) {
     memcpy(s->f[k], nf[k], sizeof(double) * Nnew);
@@ -1020,9 +1039,18 @@ static void ms_execute(or_sim *s, const int *act, const int *partner) {
   }
   memcpy(s->label, nlabel, sizeof(int) * Nnew);
   memcpy(s->level, nlevel, sizeof(int) * Nnew);
-  free(nlabel); free(nlevel); free(offset);
+  free(nlabel); free(nlevel);
   s->np = Nnew;
 }
+
+int or_ms_execute(const or_params *p, int np, const int *act,
+                  const int *partner, double *const *in, const int *label,
+                  const int *level, double **out, int *nlabel, int *nlevel) {
+  return ms_execute_core(p, np, act, partner, in, label, level, out, nlabel,
+                         nlevel);
+}
+
+int or_field_count(void) { return OR_NFIELDS; }
 
 /* ---- legacy ELFS merge/split (merge_split_ops.h:86-755), ms_mode 1 ---- */
 

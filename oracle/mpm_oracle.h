@@ -208,6 +208,13 @@ void or_ms_decide(const or_params *p, int np, double dt, const double *x,
                   const double *vpy_dx, const double *vpy_dy,
                   const double *vpx, const double *vpy, const int *level,
                   int *action, int *partner, int counts[2]);
+/* execute_merge_split (merge_split_ops_cmes.h:258-427) on arrays: in/out
+ * are or_field_count() field arrays in or_field_name() order, out, nlabel,
+ * nlevel with room for np + splits; returns the new particle count. */
+int or_ms_execute(const or_params *p, int np, const int *act,
+                  const int *partner, double *const *in, const int *label,
+                  const int *level, double **out, int *nlabel, int *nlevel);
+int or_field_count(void);
 
 #ifdef __cplusplus
 }

@@ -144,6 +144,43 @@ def elfs_cells(counts, h):
             dist[:n].reshape(ncols, nrows).T)
 
 
+def field_names():
+    """The oracle's particle fields (FIELD_NAMES order)."""
+    L = lib()
+    L.or_field_name.restype = C.c_char_p
+    L.or_field_name.argtypes = [C.c_int]
+    L.or_field_count.restype = C.c_int
+    return [L.or_field_name(k).decode() for k in range(L.or_field_count())]
+
+
+def ms_execute(prm: dict, act, partner, fields: dict, label, level):
+    """execute_merge_split (merge_split_ops_cmes.h:258-427) on a snapshot:
+    fields maps every name of field_names() to an array; returns (fields,
+    label, level) of the new particle set."""
+    names = field_names()
+    act = np.ascontiguousarray(act, np.int32)
+    n = len(act)
+    nnew = int(np.sum(np.array([1, 2, 1, 0])[act])) if n else 0
+    ins = [np.ascontiguousarray(fields[k], np.float64) for k in names]
+    outs = [np.zeros(max(nnew, 1)) for _ in names]
+    dpp = C.POINTER(C.c_double) * len(names)
+    pin = dpp(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in ins])
+    pout = dpp(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in outs])
+    nlabel = np.zeros(max(nnew, 1), np.int32)
+    nlevel = np.zeros(max(nnew, 1), np.int32)
+    P = params_struct(prm)
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    L = lib()
+    L.or_ms_execute.restype = C.c_int
+    L.or_ms_execute.argtypes = [C.POINTER(OrParams), C.c_int, ip, ip, dpp, ip, ip,
+                                dpp, ip, ip]
+    got = L.or_ms_execute(C.byref(P), n, act, np.ascontiguousarray(partner, np.int32),
+                          pin, np.ascontiguousarray(label, np.int32),
+                          np.ascontiguousarray(level, np.int32), pout, nlabel, nlevel)
+    assert got == nnew
+    return ({k: o[:nnew] for k, o in zip(names, outs)}, nlabel[:nnew], nlevel[:nnew])
+
+
 def ms_decide(prm: dict, dt, x, y, Ap, hp, vpx_dx, vpx_dy, vpy_dx, vpy_dy,
               vpx, vpy, level):
     n = len(x)

@@ -1,6 +1,8 @@
-"""Device merge/split decisions (amsc_ms_decide) against the oracle's
-restatement of decide_actions (merge_split_ops_cmes.h:116-253): identical
-action and partner arrays, with and without a binding ms_max_ops cap."""
+"""Device merge/split on the caller's arrays against the oracle:
+decisions (amsc_ms_decide, decide_actions of merge_split_ops_cmes.h:116-253)
+give identical action and partner arrays, with and without a binding
+ms_max_ops cap; execution (amsc_ms_execute, execute_merge_split :258-427)
+gives the oracle's new particle set bit for bit."""
 import numpy as np
 import pytest
 
@@ -72,3 +74,62 @@ def test_decisions_identical(seed, max_ops):
     assert np.array_equal(c1, c2)
     if max_ops == 1e9:
         assert (a1 == 1).sum() > 50 and (a1 == 2).sum() > 50  # non-trivial
+
+
+def exec_state(case, steps):
+    """An oracle state just before a merge/split, its decisions, and the
+    oracle's execute_merge_split of it."""
+    from test_oracle_step import small_slope_case
+    from amsc_amd.data import particles_from_data
+    if case == "slope":
+        d = small_slope_case()
+        d.update(MERGE_SPLIT_ON=1, ms_call_interval=4, ms_alpha=0.3, ms_beta=0.52,
+                 ms_max_dv=2.0, ms_hp_min=0.0)
+    else:
+        d = scenarios.dambreak_case()
+        d.update(ms_alpha=0.45, ms_beta=0.55, ms_max_dv=0.5, ms_hp_min=0.01,
+                 ms_shear_split=0.02)
+    prm = params_from_data(d)
+    o = oracle.OracleSim(prm, particles_from_data(d), d["Z"])
+    o.run(steps)
+    names = oracle.field_names()
+    f = {k: o.get(k) for k in names}
+    label, level = o.get_int("label"), o.get_int("level")
+    act, partner, counts = oracle.ms_decide(
+        prm, o.dt, f["x"], f["y"], f["Ap"], f["hp"], f["vpx_dx"], f["vpx_dy"],
+        f["vpy_dx"], f["vpy_dy"], f["vpx"], f["vpy"], level)
+    return prm, names, f, label, level, act, partner, counts
+
+
+@pytest.mark.parametrize("case,steps", [("slope", 7), ("dambreak", 30)])
+def test_execute_matches_oracle(case, steps):
+    """amsc_ms_execute (execute_merge_split on the caller's arrays, all 35
+    of the reference's particle fields) equals the oracle's execution of
+    the same decisions bit for bit: fields, labels, levels, order."""
+    from amsc_amd import native
+    D = native.DeviceArray
+    print(f"[ms] execute {case}", flush=True)
+    prm, names, f, label, level, act, partner, counts = exec_state(case, steps)
+    assert counts[0] > 0 and (case == "slope" or counts[1] > 0), counts
+    want, wl, wv = oracle.ms_execute(prm, act, partner, f, label, level)
+    n = len(act)
+    cap = n + int(np.sum(act == 1)) + 7
+    keep = [D(f[k]) for k in names]
+    outs = [D.zeros(cap, np.float64) for _ in names]
+    da, dp = D(act, np.int32), D(partner, np.int32)
+    dl, dv = D(label, np.int32), D(level, np.int32)
+    ol, ov = D.zeros(cap, np.int32), D.zeros(cap, np.int32)
+    nnew = native.ms_execute(prm, n, da.ptr, dp.ptr, names, [a.ptr for a in keep],
+                             [a.ptr for a in outs], dl.ptr, dv.ptr, ol.ptr, ov.ptr,
+                             cap)
+    assert nnew == len(want["x"])
+    for k, a in zip(names, outs):
+        got = a.numpy()[:nnew]
+        assert np.array_equal(got.view(np.int64), want[k].view(np.int64)), k
+    assert np.array_equal(ol.numpy()[:nnew], wl)
+    assert np.array_equal(ov.numpy()[:nnew], wv)
+    # a capacity one short is refused
+    with pytest.raises(native.AmscError):
+        native.ms_execute(prm, n, da.ptr, dp.ptr, names, [a.ptr for a in keep],
+                          [a.ptr for a in outs], dl.ptr, dv.ptr, ol.ptr, ov.ptr,
+                          nnew - 1)
