@@ -13,6 +13,8 @@
 #include "amsc_mpm.h"
 
 #include <climits>
+#include <cmath>
+#include <algorithm>
 #include <mutex>
 
 namespace amsc {
@@ -256,6 +258,66 @@ int tiled_gather(int nrows, double hx, double hy, const int *cstart,
                                                  cstart, cptcls, colour, x, y, f);
   AMSC_LAUNCH_CHECK();
   return AMSC_OK;
+}
+
+// ---- CFL maxima (gpu_offload.cpp:285-292): max |vpx|, max |vpy|, max hp.
+// Each workgroup writes its three maxima, one workgroup reduces them (max is
+// exact in any order).
+constexpr int kCflBlock = 256;
+
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const double o = __shfl_xor(v, d);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ void block_max3(double v[3], double (*s)[kCflBlock / 64]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    v[k] = wave_max_d(v[k]);
+    if (lane == 0) s[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double m = s[k][0];
+    for (int i = 1; i < kCflBlock / 64; ++i) m = s[k][i] > m ? s[k][i] : m;
+    v[k] = m;
+  }
+}
+
+__global__ void __launch_bounds__(kCflBlock)
+    cfl_partial_kernel(int np, const double *__restrict__ vpx,
+                       const double *__restrict__ vpy,
+                       const double *__restrict__ hp, double *part) {
+  __shared__ double s[3][kCflBlock / 64];
+  double v[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int i = blockIdx.x * kCflBlock + threadIdx.x; i < np;
+       i += gridDim.x * kCflBlock) {
+    const double a = fabs(vpx[i]), b = fabs(vpy[i]), h = hp[i];
+    v[0] = a > v[0] ? a : v[0];
+    v[1] = b > v[1] ? b : v[1];
+    v[2] = h > v[2] ? h : v[2];
+  }
+  block_max3(v, s);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = v[k];
+}
+
+__global__ void __launch_bounds__(kCflBlock)
+    cfl_final_kernel(int nb, const double *__restrict__ part, double *out) {
+  __shared__ double s[3][kCflBlock / 64];
+  double v[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int b = threadIdx.x; b < nb; b += kCflBlock)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = part[3 * b + k] > v[k] ? part[3 * b + k] : v[k];
+  block_max3(v, s);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) out[k] = v[k];
 }
 
 struct Ptrs25 {
@@ -792,6 +854,36 @@ int amsc_normalize_plotvars(int nn, double *d_pv_rho, double *d_pv_vvx,
   if (int rc = check_n(nn, "nn")) return rc;
   LAUNCH1D(normalize_kernel, nn, nn, d_pv_rho, d_pv_vvx, d_pv_vvy, d_pv_avx,
            d_pv_avy);
+  return AMSC_OK;
+}
+
+int amsc_cfl_dt(int np, const double *d_vpx, const double *d_vpy,
+                const double *d_hp, double g, double cfl, double hx, double hy,
+                int it, double *dt, double *cel_out) {
+  if (np <= 0 || !d_vpx || !d_vpy || !d_hp || !dt)
+    return fail(AMSC_ERR_INVALID, "amsc_cfl_dt: bad arguments");
+  const int nb = std::min((int)grid_for(np, kCflBlock), 1024);
+  double *part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * (3 * nb + 3)) != hipSuccess)
+    return fail(AMSC_ERR_NOMEM, "amsc_cfl_dt: workspace allocation");
+  hipStream_t st = current_stream();
+  cfl_partial_kernel<<<nb, kCflBlock, 0, st>>>(np, d_vpx, d_vpy, d_hp, part);
+  cfl_final_kernel<<<1, kCflBlock, 0, st>>>(nb, part, part + 3 * nb);
+  double m[3] = {0, 0, 0};
+  int rc = AMSC_OK;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(m, part + 3 * nb, sizeof(m), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    rc = fail(AMSC_ERR_HIP, "amsc_cfl_dt: reduction failed");
+  (void)hipFree(part);
+  if (rc) return rc;
+  // gpu_offload.cpp:293-298
+  const double max_vx = m[0], max_vy = m[1], hmax = m[2];
+  const double a = std::sqrt(g * hmax) + max_vx, b = std::sqrt(g * hmax) + max_vy;
+  const double cel = std::fabs(std::max(a, b));
+  if (cel_out) *cel_out = cel;
+  if (it > 0) *dt = cfl * std::min(hx, hy) / (1e-2 + cel);
   return AMSC_OK;
 }
 

@@ -1,9 +1,10 @@
 // offload_loop.cpp -- the time loop of the reference's gpu_offload driver
-// (main/gpu_offload.cpp:276-575, fixed dt, merge/split off) written against
+// (main/gpu_offload.cpp:276-575, merge/split off) written against
 // the kernel-level drop-in include/gpu_kernels.h: the same eighteen
 // gpu_kernels:: calls in the same order (K1-K10), with the host CSR rebuild
 // after each step (init_particle_mesh, gpu_offload.cpp:551-562) replaced by
-// amsc_bin_particles.  It is the worked example of keeping gpu_offload's
+// amsc_bin_particles and the adaptive dt (host max_element over copies of
+// the particle arrays, :282-301) by amsc_cfl_dt.  It is the worked example of keeping gpu_offload's
 // loop and swapping its kernels, and a test driver: state in and out as
 // binary files.
 //
@@ -11,6 +12,7 @@
 //
 // in.bin:  int32  np nrows ncols nsteps bc_flag
 //          float64 hx hy dt g rho mu tau_Y cc A_coeff C_coeff tan_fa xi fric_on
+//                  cfl  (dt > 0: fixed dt; dt <= 0: adaptive, CFL = cfl)
 //          float64 particle arrays (np each, kParticle order), Z (nn)
 // out.bin: float64 particle arrays (kParticle order), nodal arrays
 //          (kNodal order), then int32 p2g (np), cstart (nc+1), cptcls (np)
@@ -66,11 +68,14 @@ int main(int argc, char **argv) {
   rd(in, hdr, 5);
   const int np = hdr[0], nrows = hdr[1], ncols = hdr[2], nsteps = hdr[3];
   const bool bc_flag = hdr[4] != 0;
-  double s[13];
-  rd(in, s, 13);
-  const double hx = s[0], hy = s[1], dt = s[2], g = s[3], rho = s[4], mu = s[5],
-               tau_Y = s[6], cc = s[7], A_coeff = s[8], C_coeff = s[9],
-               tan_fa = s[10], xi = s[11], fric_on = s[12];
+  double s[14];
+  rd(in, s, 14);
+  const double hx = s[0], hy = s[1], dt_fixed = s[2], g = s[3], rho = s[4],
+               mu = s[5], tau_Y = s[6], cc = s[7], A_coeff = s[8],
+               C_coeff = s[9], tan_fa = s[10], xi = s[11], fric_on = s[12],
+               cfl = s[13];
+  const bool fixed = dt_fixed > 0.0;
+  double dt = fixed ? dt_fixed : 1.0e-3;  // gpu_offload.cpp:255
   const int nn = (nrows + 1) * (ncols + 1), nc = nrows * ncols;
   std::vector<double> host((size_t)np);
   double *p[NP_FIELDS], *v[NN_FIELDS];
@@ -96,6 +101,11 @@ int main(int argc, char **argv) {
   ok(amsc_synchronize(), "synchronize");
   const auto t0 = std::chrono::steady_clock::now();
   for (int it = 0; it < nsteps; ++it) {
+    // dt, gpu_offload.cpp:282-301 (it = steps taken so far)
+    if (!fixed)
+      ok(amsc_cfl_dt(np, p[VPX], p[VPY], p[HP], g, cfl, hx, hy, it, &dt,
+                     nullptr),
+         "cfl");
     // K1, gpu_offload.cpp:410-422
     gk::reset_nodal_arrays(nn, v[MV], v[MOMVX], v[MOMVY], v[FEXTX], v[FEXTY],
                            v[FINTX], v[FINTY], v[FTOTX], v[FTOTY], v[AVX],

@@ -120,6 +120,9 @@ def lib():
                                      _ip]
     L.amsc_ms_decide.argtypes = [C.POINTER(AmscParams), C.c_int, C.c_double] + \
         [_vp] * 15 + [_ip]
+    L.amsc_cfl_dt.argtypes = [C.c_int, _vp, _vp, _vp, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_int,
+                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.amsc_ms_execute.argtypes = [C.POINTER(AmscParams), C.c_int, _vp, _vp, C.c_int,
                                   C.POINTER(C.c_char_p), C.POINTER(_vp),
                                   C.POINTER(_vp), _vp, _vp, _vp, _vp, C.c_int,
@@ -211,6 +214,15 @@ def ms_decide(prm: dict, np_, dt, ptrs, d_level, d_cstart, d_cptcls,
     check(lib().amsc_ms_decide(C.byref(P), np_, dt, *ptrs, d_level, d_cstart,
                                d_cptcls, d_action, d_partner, counts))
     return counts
+
+
+def cfl_dt(np_, d_vpx, d_vpy, d_hp, g, cfl, hx, hy, it, dt):
+    """amsc_cfl_dt: (dt after the call, cel)."""
+    d = C.c_double(dt)
+    cel = C.c_double(0.0)
+    check(lib().amsc_cfl_dt(np_, d_vpx, d_vpy, d_hp, g, cfl, hx, hy, it,
+                            C.byref(d), C.byref(cel)))
+    return d.value, cel.value
 
 
 def ms_execute(prm: dict, np_, d_action, d_partner, names, d_in, d_out,

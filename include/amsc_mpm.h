@@ -191,6 +191,17 @@ int amsc_normalize_plotvars(int nn, double *d_pv_rho, double *d_pv_vvx,
                             double *d_pv_vvy, double *d_pv_avx,
                             double *d_pv_avy);
 
+/* The adaptive time step of gpu_offload.cpp:282-301 (optim_par.cpp:285-307)
+ * with the maxima taken on the device instead of over host copies of the
+ * particle arrays: cel = |max(sqrt(g*hmax) + max|vpx|, sqrt(g*hmax) +
+ * max|vpy|)| over np > 0 particles; when it > 0, *dt = cfl*min(hx,hy) /
+ * (1e-2 + cel), otherwise *dt is left as it is (the first step keeps its
+ * dt, gpu_offload.cpp:255).  *cel_out (may be NULL) receives cel.  Host
+ * scalars; synchronous. */
+int amsc_cfl_dt(int np, const double *d_vpx, const double *d_vpy,
+                const double *d_hp, double g, double cfl, double hx, double hy,
+                int it, double *dt, double *cel_out);
+
 /* ------------------------------------------------------------------ 2. */
 /* Replaces quadgrid particles_t::init_particle_mesh() (called at
  * optim_par.cpp:341 and gpu_offload.cpp:243,380,549, rebuilt on the host

@@ -97,3 +97,35 @@ def test_bad_colouring_is_rejected():
     bad["coloff"] = np.array([0, 5, 9, 13, case["nrows"] * case["ncols"] + 1], np.int32)
     with pytest.raises(native.AmscError):
         run_device(name, bad)
+
+
+def test_cfl_dt_matches_the_oracle_adaptive_step():
+    """amsc_cfl_dt on the state after step 1 gives the dt the oracle's
+    adaptive step 2 takes (gpu_offload.cpp:282-301, optim_par.cpp:285-307),
+    bit for bit; the first step keeps its dt; the maxima are of |v| and of
+    hp, whatever the signs."""
+    from amsc_amd import native
+    from amsc_amd.data import params_from_data, particles_from_data
+    from test_oracle_step import small_slope_case
+    print("[dropin] cfl dt", flush=True)
+    d = small_slope_case()
+    d.update(DT_FIXED=0.0, CFL=0.4, T=1.0)
+    prm = params_from_data(d)
+    o = oracle.OracleSim(prm, particles_from_data(d), d["Z"])
+    assert o.step()
+    arrs = [native.DeviceArray(o.get(k)) for k in ("vpx", "vpy", "hp")]
+    assert o.step()
+    dt, cel = native.cfl_dt(len(o.get("x")), *[a.ptr for a in arrs], prm["g"],
+                            prm["cfl"], prm["hx"], prm["hy"], 1, 7.0)
+    assert np.float64(dt).view(np.int64) == np.float64(o.dt).view(np.int64)
+    dt0, _ = native.cfl_dt(len(o.get("x")), *[a.ptr for a in arrs], prm["g"],
+                           prm["cfl"], prm["hx"], prm["hy"], 0, 1e-3)
+    assert dt0 == 1e-3
+    rng = np.random.default_rng(4)
+    for n in (1, 255, 70001):
+        vx, vy, h = rng.normal(0, 3, n), rng.normal(0, 3, n), rng.uniform(0, 5, n)
+        vx[n // 2] = -40.0
+        a = [native.DeviceArray(v) for v in (vx, vy, h)]
+        _, cel = native.cfl_dt(n, *[t.ptr for t in a], 9.81, 0.4, 1.0, 1.0, 3, 1.0)
+        sq = np.sqrt(9.81 * h.max())
+        assert cel == abs(max(sq + np.abs(vx).max(), sq + np.abs(vy).max()))

@@ -1,6 +1,6 @@
 """gpu_offload's time loop kept, its kernels swapped: amsc_offload_loop runs
 the reference driver's step (main/gpu_offload.cpp:410-562: K1-K10 and the
-CSR rebuild, fixed dt, merge/split off) through the C++ drop-in header
+CSR rebuild, fixed or adaptive dt, merge/split off) through the C++ drop-in header
 include/gpu_kernels.h on the device.  The same loop restated here over the
 oracle's kernels (each pinned bit for bit to the reference's compiled
 main/gpu_kernels.cpp) and, where that build is present, over the
@@ -60,7 +60,7 @@ def initial_state(d, dt):
               hy=prm["hy"], dt=dt, g=prm["g"], rho=prm["rho"], mu=prm["mu"],
               tau_Y=prm["tauy"], cc=prm["bingham_on"], A_coeff=3.0 / 2.0,
               C_coeff=65.0 / 32.0, tan_fa=math.tan(prm["phi"] * math.pi / 180.0),
-              xi=prm["xi"], fric_on=prm["friction_on"], bc_flag=1)
+              xi=prm["xi"], fric_on=prm["friction_on"], bc_flag=1, cfl=prm["cfl"])
     for k in ("min_sc_out", "min_hp_out", "max_coeff_out", "max_nv_out",
               "max_F11_out", "max_dev_out", "max_pres_out", "max_Dxx_out"):
         st[k] = np.zeros(1)
@@ -73,9 +73,22 @@ def rebin(st):
     st.update(p2g=p2g, cstart=cstart, cptcls=cptcls, ccidx=ccidx, coloff=coloff)
 
 
-def host_loop(st, nsteps, kernel):
+def cfl_dt(st, it):
+    """gpu_offload.cpp:282-301: dt from the particle maxima (it > 0)."""
+    hmax = st["hp"].max()
+    sq = math.sqrt(st["g"] * hmax)
+    cel = abs(max(sq + np.abs(st["vpx"]).max(), sq + np.abs(st["vpy"]).max()))
+    if it > 0:
+        st["dt"] = st["cfl"] * min(st["hx"], st["hy"]) / (1e-2 + cel)
+
+
+def host_loop(st, nsteps, kernel, adaptive=False):
     rebin(st)
-    for _ in range(nsteps):
+    if adaptive:
+        st["dt"] = 1e-3  # gpu_offload.cpp:255
+    for it in range(nsteps):
+        if adaptive:
+            cfl_dt(st, it)
         for name in STEP:
             args, outs = host_args(name, st)
             kernel(name)(*args)
@@ -92,7 +105,7 @@ def device_loop(st, nsteps, tmp_path):
                  np.int32).tofile(f)
         np.array([st[k] for k in ("hx", "hy", "dt", "g", "rho", "mu", "tau_Y",
                                   "cc", "A_coeff", "C_coeff", "tan_fa", "xi",
-                                  "fric_on")], np.float64).tofile(f)
+                                  "fric_on", "cfl")], np.float64).tofile(f)
         for k in PARTICLE:
             np.ascontiguousarray(st[k], np.float64).tofile(f)
         np.ascontiguousarray(st["Z"], np.float64).tofile(f)
@@ -125,6 +138,8 @@ CASES = {
     "slope": (small_slope_case, 25, 2e-3),
     # channel dam break (dambreak.m): fast enough that particles change cells
     "dambreak": (scenarios.dambreak_case, 40, 1e-3),
+    # the same dam break with gpu_offload's adaptive dt (dt <= 0 selects it)
+    "dambreak_adaptive": (scenarios.dambreak_case, 40, 0.0),
 }
 
 
@@ -141,13 +156,13 @@ def test_offload_loop_through_the_dropin_header(case, checker, tmp_path):
                               st["hx"], st["hy"])[0]
     got = device_loop(st, nsteps, tmp_path)
     kern = oracle.oracle_kernel if checker == "oracle" else oracle.ref_kernel
-    want = host_loop(st, nsteps, kern)
+    want = host_loop(st, nsteps, kern, adaptive=dt <= 0.0)
     for k in PARTICLE + NODAL:
         assert np.array_equal(bits(got[k]), bits(want[k])), k
     for k in ("p2g", "cstart", "cptcls"):
         assert np.array_equal(got[k], np.asarray(want[k], np.int32)), k
     assert not np.isnan(got["x"]).any()
-    if case == "dambreak":
+    if case.startswith("dambreak"):
         assert np.count_nonzero(got["p2g"] != p0) > 100  # particles changed cells
     else:
         assert np.any(got["Fric_x"] != 0.0)  # friction acted

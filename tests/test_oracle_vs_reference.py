@@ -70,7 +70,7 @@ def test_scatter_colour_loop_touches_every_node_once_per_cell():
                     assert colour[cell] != colour[(r + dr) + (c + dc) * nrows]
 
 
-@pytest.mark.parametrize("case", ["slope", "dambreak"])
+@pytest.mark.parametrize("case", ["slope", "dambreak", "dambreak_adaptive"])
 def test_offload_loop_composition_matches_reference_build(case):
     """gpu_offload's step (gpu_offload.cpp:410-562, fixed dt) composed from
     the oracle's kernels equals the same composition over the reference's
@@ -81,8 +81,8 @@ def test_offload_loop_composition_matches_reference_build(case):
     from test_gpu_offload_loop import CASES, NODAL, PARTICLE, host_loop, initial_state
     make, nsteps, dt = CASES[case]
     d = make()
-    a = host_loop(initial_state(d, dt), nsteps, oracle.oracle_kernel)
-    b = host_loop(initial_state(d, dt), nsteps, oracle.ref_kernel)
+    a = host_loop(initial_state(d, dt), nsteps, oracle.oracle_kernel, dt <= 0.0)
+    b = host_loop(initial_state(d, dt), nsteps, oracle.ref_kernel, dt <= 0.0)
     for k in PARTICLE + NODAL:
         assert np.array_equal(a[k].view(np.int64), b[k].view(np.int64)), k
     for k in ("p2g", "cstart", "cptcls"):
