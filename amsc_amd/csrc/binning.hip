@@ -5,9 +5,9 @@
 // Pipeline (all integer, so results are exact and run-to-run identical):
 //   1. key = cell of the particle, slot = atomicAdd(count[key]) aggregated
 //      per wavefront (fused into the particle-update kernel of a time step);
-//   2. cstart = exclusive scan of count (reduce-then-scan: tile sums, one
-//      workgroup scans the tile sums, every tile scans its values from its
-//      offset; no look-back state to initialise, no waits between
+//   2. cstart = exclusive scan of count (reduce-then-scan: tile sums, then
+//      every tile adds up the sums before it and scans its values from
+//      there; no look-back state to initialise, no waits between
 //      workgroups), the counts reset to 0 for the next binning as they are
 //      read;
 //   3. cptcls[cstart[key] + slot] = id;
@@ -144,35 +144,27 @@ __global__ void __launch_bounds__(kScanThreads)
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
 
-// one workgroup: tile_sum[0..nt) becomes its exclusive prefix
-__global__ void __launch_bounds__(kScanThreads)
-    scan_tile_sums_kernel(int *tile_sum, int nt) {
-  __shared__ int s_w[17];
-  int carry = 0;
-  for (int t0 = 0; t0 < nt; t0 += kScanThreads) {
-    const int i = t0 + threadIdx.x;
-    const int v = i < nt ? tile_sum[i] : 0;
-    int total;
-    const int ex = block_excl_scan(v, s_w, &total);
-    if (i < nt) tile_sum[i] = carry + ex;
-    carry += total;
-    __syncthreads();
-  }
-}
-
 // out[i+1] = prefix through in[i]; out[0] = 0; clear != null: clear[i] = 0
-// after in[i] is read (clear == in for the binning counts).
+// after in[i] is read (clear == in for the binning counts).  Each tile adds
+// up the sums of the tiles before it itself (a few hundred cached reads),
+// which spares a one-workgroup pass between the two kernels.
 __global__ void __launch_bounds__(kScanThreads)
-    scan_apply_kernel(const int *in, int n, const int *__restrict__ tile_off,
+    scan_apply_kernel(const int *in, int n, const int *__restrict__ tile_sum,
                       int *out, int *clear) {
   __shared__ int s_w[17];
+  int pre = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads)
+    pre += tile_sum[b];
+  int tile_off;
+  (void)block_excl_scan(pre, s_w, &tile_off);
+  __syncthreads();  // s_w is reused below
   const long long base =
       (long long)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int v[4];
   load4(in, base, n, v);
   int total;
   const int ex = block_excl_scan(v[0] + v[1] + v[2] + v[3], s_w, &total);
-  int run = tile_off[blockIdx.x] + ex;
+  int run = tile_off + ex;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (base + k < n) {
@@ -208,7 +200,6 @@ int scan_launch(const int *in, int *out, int n, ScanWork &w, hipStream_t st,
   const int nt = scan_tiles(n);
   int *ts = static_cast<int *>(w.tmp);
   scan_sums_kernel<<<nt, kScanThreads, 0, st>>>(in, n, ts);
-  scan_tile_sums_kernel<<<1, kScanThreads, 0, st>>>(ts, nt);
   scan_apply_kernel<<<nt, kScanThreads, 0, st>>>(in, n, ts, out, clear);
   AMSC_LAUNCH_CHECK();
   return AMSC_OK;
