@@ -139,8 +139,10 @@ def field_errors(a, b, prm):
     new particles label -1 and may decide differently in the two runs; the
     reference's own CPU/GPU comparison matches by label the same way):
     depth relative to b's maximum depth (max, p99, median, fraction within
-    1e-4) and velocity relative to sqrt(g hmax).  Field: the nodal depth
-    (nodal_depth), max and relative L2 error."""
+    1e-4), velocity relative to sqrt(g hmax), position relative to the
+    cell size, stress (F_11, F_12, F_22) relative to b's largest stress
+    component.  Field: the nodal depth (nodal_depth), max and relative L2
+    error."""
     ga = nodal_depth(a["x"], a["y"], a["Vp"], prm)
     gb = nodal_depth(b["x"], b["y"], b["Vp"], prm)
     gmax = max(gb.max(), 1e-30)
@@ -162,12 +164,25 @@ def field_errors(a, b, prm):
                     "hp_frac_within_1e-4": float(np.mean(dh <= 1e-4)),
                     "v_max_over_sqrt_gh": float(dv.max()),
                     "v_median_over_sqrt_gh": float(np.median(dv))})
+        h = min(prm["hx"], prm["hy"])
+        dx = np.hypot(a["x"][ia] - b["x"][ib], a["y"][ia] - b["y"][ib]) / h
+        out.update({"pos_max_over_h": float(dx.max()),
+                    "pos_median_over_h": float(np.median(dx))})
+        fk = [k for k in ("F_11", "F_12", "F_22") if k in a and k in b]
+        if fk:
+            fmax = max(max(np.abs(b[k]).max() for k in fk), 1e-30)
+            dF = np.max([np.abs(a[k][ia] - b[k][ib]) for k in fk], axis=0) / fmax
+            out.update({"F_max": float(dF.max()), "F_median": float(np.median(dF))})
     return out
+
+
+# particle state compared between precisions (the persistent fields)
+STATE_FIELDS = ("x", "y", "Vp", "hp", "vpx", "vpy", "F_11", "F_12", "F_22")
 
 
 def grab(sim, k=None):
     g = (lambda f: sim.get(f)) if k is None else (lambda f: sim.get(k, f))
-    out = {f: g(f) for f in ("x", "y", "Vp", "hp", "vpx", "vpy")}
+    out = {f: g(f) for f in STATE_FIELDS}
     out["label"] = sim.get_int("label") if k is None else sim.get_int(k, "label")
     return out
 
@@ -339,7 +354,7 @@ def still_water_bench(args):
 
 def served_bench(args):
     from amsc_amd import scenarios, serve
-    FIELDS = ("x", "y", "Vp", "hp", "vpx", "vpy", "label")
+    FIELDS = STATE_FIELDS + ("label",)
     out = {}
     # the three arrival rates of the served configuration, then a saturated
     # queue (every request waiting at t = 0) that measures service capacity

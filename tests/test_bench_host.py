@@ -36,7 +36,8 @@ def particles(seed, n=300):
     y = rng.uniform(0, PRM["nrows"] * PRM["hy"], n)
     return {"x": x, "y": y, "Vp": rng.uniform(0.1, 1.0, n), "hp": rng.uniform(0.5, 2.0, n),
             "vpx": rng.normal(0, 1, n), "vpy": rng.normal(0, 1, n),
-            "label": np.arange(n, dtype=np.int32)}
+            "F_11": rng.normal(0, 50, n), "F_12": rng.normal(0, 20, n),
+            "F_22": rng.normal(0, 50, n), "label": np.arange(n, dtype=np.int32)}
 
 
 def test_nodal_depth_conserves_volume_and_is_continuous():
@@ -73,6 +74,14 @@ def test_field_errors_match_particles_by_label():
     e = bench.field_errors(d, a, PRM)
     assert e["hp_max"] == pytest.approx(1e-3 / a["hp"].max(), rel=1e-9)
     assert e["hp_frac_within_1e-4"] == pytest.approx(1 - 1 / len(a["x"]))
+    # position error in cells, stress error against the largest component
+    d = {k: v.copy() for k, v in a.items()}
+    d["x"][7] += 0.1 * PRM["hy"]
+    d["F_12"][9] += 2.0
+    e = bench.field_errors(d, a, PRM)
+    assert e["pos_max_over_h"] == pytest.approx(0.1 * PRM["hy"] / min(PRM["hx"], PRM["hy"]))
+    fmax = max(np.abs(a[k]).max() for k in ("F_11", "F_12", "F_22"))
+    assert e["F_max"] == pytest.approx(2.0 / fmax, rel=1e-9) and e["F_median"] == 0.0
 
 
 def test_aggregate_errors():
