@@ -449,8 +449,9 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, "
-              f"WORLD_SIZE {world}): the MPM step has no multi-GPU path here; "
-              "see DESIGN.md for how it would shard.", file=sys.stderr)
+              f"WORLD_SIZE {world}): the MPM step has no multi-GPU path here "
+              "(one process per GPU, each owning a column strip of the grid, "
+              "is the sharding it would take).", file=sys.stderr)
         sys.exit(2)
 
     d, desc = workload(args.config)

@@ -8,9 +8,9 @@
  * returning (the reference's OpenMP target regions are synchronous); an
  * error prints amsc_last_error() and aborts, where the reference would abort
  * inside the offload runtime.  Pointers are device addresses (in an OpenMP
- * offload program: `#pragma omp target data use_device_addr(...)`, see
- * INTEGRATION.md); `coloff` stays a host array and the *_out diagnostics are
- * ignored, as in the reference.
+ * offload program, the ones `#pragma omp target data use_device_addr(...)`
+ * hands out for the mapped arrays); `coloff` stays a host array and the
+ * *_out diagnostics are ignored, as in the reference.
  */
 #ifndef GPU_KERNELS_H
 #define GPU_KERNELS_H
