@@ -15,7 +15,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from amsc_amd import scenarios
+from amsc_amd import offload_input, scenarios
 from amsc_amd.data import params_from_data, particles_from_data
 from kernel_cases import ARGS, host_args
 from oracle import oracle
@@ -28,10 +28,7 @@ LOOP = os.path.join(os.path.dirname(__file__), "..", "amsc_amd", "_lib",
 
 # particle arrays in the driver's order (kernel argument names) and the
 # oracle simulation's names for them
-PARTICLE = ["x", "y", "Mp", "Vp", "Ap", "hp", "H", "vpx", "vpy", "mom_px",
-            "mom_py", "Fb_x", "Fb_y", "F11", "F12", "F21", "F22", "Fric_px",
-            "Fric_py", "dZxp", "dZyp", "Fpx", "Fpy", "apx", "apy", "vpxL",
-            "vpyL", "vpx_dx", "vpx_dy", "vpy_dx", "vpy_dy", "Zp", "hpZ"]
+PARTICLE = offload_input.PARTICLE
 ORACLE_NAME = {"F11": "F_11", "F12": "F_12", "F21": "F_21", "F22": "F_22"}
 NODAL = ["Mv", "mom_vx", "mom_vy", "F_ext_vx", "F_ext_vy", "F_int_vx",
          "F_int_vy", "Ftot_vx", "Ftot_vy", "avx", "avy", "vvx", "vvy", "vvxL",
@@ -56,11 +53,8 @@ def initial_state(d, dt):
     nn = (nrows + 1) * (ncols + 1)
     st.update({k: np.zeros(nn) for k in NODAL})
     st["Z"] = np.asarray(d["Z"], np.float64)
-    st.update(np=len(st["x"]), nn=nn, nrows=nrows, ncols=ncols, hx=prm["hx"],
-              hy=prm["hy"], dt=dt, g=prm["g"], rho=prm["rho"], mu=prm["mu"],
-              tau_Y=prm["tauy"], cc=prm["bingham_on"], A_coeff=3.0 / 2.0,
-              C_coeff=65.0 / 32.0, tan_fa=math.tan(prm["phi"] * math.pi / 180.0),
-              xi=prm["xi"], fric_on=prm["friction_on"], bc_flag=1, cfl=prm["cfl"])
+    st.update(np=len(st["x"]), nn=nn, nrows=nrows, ncols=ncols, bc_flag=1)
+    st.update(offload_input.scalars(prm, dt))
     for k in ("min_sc_out", "min_hp_out", "max_coeff_out", "max_nv_out",
               "max_F11_out", "max_dev_out", "max_pres_out", "max_Dxx_out"):
         st[k] = np.zeros(1)
@@ -100,15 +94,7 @@ def host_loop(st, nsteps, kernel, adaptive=False):
 def device_loop(st, nsteps, tmp_path):
     np_, nn, nc = st["np"], st["nn"], st["nrows"] * st["ncols"]
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
-    with open(src, "wb") as f:
-        np.array([np_, st["nrows"], st["ncols"], nsteps, st["bc_flag"]],
-                 np.int32).tofile(f)
-        np.array([st[k] for k in ("hx", "hy", "dt", "g", "rho", "mu", "tau_Y",
-                                  "cc", "A_coeff", "C_coeff", "tan_fa", "xi",
-                                  "fric_on", "cfl")], np.float64).tofile(f)
-        for k in PARTICLE:
-            np.ascontiguousarray(st[k], np.float64).tofile(f)
-        np.ascontiguousarray(st["Z"], np.float64).tofile(f)
+    offload_input.write(src, st, nsteps)
     r = subprocess.run([os.path.abspath(LOOP), str(src), str(dst)],
                        capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
@@ -166,3 +152,27 @@ def test_offload_loop_through_the_dropin_header(case, checker, tmp_path):
         assert np.count_nonzero(got["p2g"] != p0) > 100  # particles changed cells
     else:
         assert np.any(got["Fric_x"] != 0.0)  # friction acted
+
+
+def test_input_writer_state_matches_the_oracle(tmp_path):
+    """amsc_amd.offload_input (which writes the loop's input from the device
+    library's step-0 initialisation, for timing and profiling the loop)
+    starts from the oracle's step-0 state in every array the loop reads
+    before writing it, and the loop then gives the oracle's bits."""
+    print("[offload input] slope", flush=True)
+    d = small_slope_case()
+    dev = offload_input.initial_state(d, 1e-3)
+    ref = initial_state(d, 1e-3)
+    read_first = ["x", "y", "Mp", "Vp", "Ap", "hp", "H", "vpx", "vpy", "mom_px",
+                  "mom_py", "Fb_x", "Fb_y", "F11", "F12", "F21", "F22",
+                  "Fric_px", "Fric_py"]
+    for k in read_first:
+        assert np.array_equal(bits(dev[k]), bits(ref[k])), k
+    for k in offload_input.SCALARS + ["nrows", "ncols", "bc_flag"]:
+        assert dev[k] == ref[k], k
+    ref.update({k: dev[k] for k in PARTICLE if k not in read_first})
+    dev.update(np=ref["np"], nn=ref["nn"])
+    got = device_loop(dev, 5, tmp_path)
+    want = host_loop(ref, 5, oracle.oracle_kernel)
+    for k in ("x", "y", "hp", "vpx", "vpy", "F11", "F22", "Zp"):
+        assert np.array_equal(bits(got[k]), bits(want[k])), k
