@@ -8,7 +8,11 @@
 // loop and swapping its kernels, and a test driver: state in and out as
 // binary files.
 //
-//   amsc_offload_loop <in.bin> <out.bin>
+//   amsc_offload_loop <in.bin> <out.bin> [--async]
+//
+// --async: gpu_kernels::set_synchronous(false), the calls only enqueue their
+// kernels (the default waits after each call, as the reference's target
+// regions do); the results are the same bits.
 //
 // in.bin:  int32  np nrows ncols nsteps bc_flag
 //          float64 hx hy dt g rho mu tau_Y cc A_coeff C_coeff tan_fa xi fric_on
@@ -19,6 +23,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "gpu_kernels.h"
@@ -58,10 +63,13 @@ template <typename T> void rd(FILE *f, T *v, size_t n) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: amsc_offload_loop <in.bin> <out.bin>\n");
+  const bool async = argc == 4 && std::string(argv[3]) == "--async";
+  if (argc != 3 && !async) {
+    std::fprintf(stderr,
+                 "usage: amsc_offload_loop <in.bin> <out.bin> [--async]\n");
     return 2;
   }
+  gpu_kernels::set_synchronous(!async);
   FILE *in = std::fopen(argv[1], "rb");
   if (!in) die("cannot open the input file");
   int hdr[5];

@@ -23,14 +23,31 @@
 namespace gpu_kernels {
 
 namespace detail {
+// Whether each call waits for its kernels before returning (the default:
+// the reference's target regions are synchronous) or only enqueues them on
+// the library's stream (set_synchronous(false)).
+inline bool &synchronous() {
+  static bool on = true;
+  return on;
+}
+
 inline void check(int rc, const char *what) {
-  if (rc == AMSC_OK) rc = amsc_synchronize();
+  if (rc == AMSC_OK && synchronous()) rc = amsc_synchronize();
   if (rc != AMSC_OK) {
     std::fprintf(stderr, "gpu_kernels::%s: %s\n", what, amsc_last_error());
     std::abort();
   }
 }
 }  // namespace detail
+
+// No counterpart in the reference.  set_synchronous(false) lets the calls
+// return once their kernels are enqueued, in order, on the library's stream:
+// for callers that pass the results of one call to the next on the device
+// only and read them on the host after amsc_synchronize().  A kernel that
+// fails while running is then reported by that later synchronisation (or
+// by the next call that has to wait, such as amsc_bin_particles or
+// amsc_cfl_dt), not by the call that launched it.
+inline void set_synchronous(bool on) { detail::synchronous() = on; }
 
 inline void reset_nodal_arrays(
     int nn, double *d_Mv, double *d_mom_vx, double *d_mom_vy,
