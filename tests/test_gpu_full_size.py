@@ -1,0 +1,77 @@
+"""The 4M-particle configuration at its full size (2048 x 2048 cells,
+hx = 0.5, fractal slope, polygonal release; amsc_amd.scenarios.large_fractal):
+the first step bit for bit against the oracle, then properties that do not
+depend on size — P2G mass and momentum transfer to round-off every step
+(data/conservation_test/scripts/conservation_check.py:36, 1e-10), total
+particle mass unchanged through the merge/split steps (steps 10 and 20), and
+the lake at rest over the same topography staying at rest
+(data/well_balance/scripts/wb_error.py:72, 1e-10)."""
+import numpy as np
+import pytest
+
+from amsc_amd import scenarios
+from amsc_amd.data import params_from_data, particles_from_data
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.int64)
+
+
+def test_large_fractal_4m():
+    from amsc_amd.simulation import Simulation
+    print("[full size] large fractal 4M", flush=True)
+    d = scenarios.large_fractal(nsteps=30)
+    prm = params_from_data(d)
+    part = particles_from_data(d)
+    assert len(part["x"]) >= 3_900_000
+    g = Simulation(prm, part, d["Z"], capacity=3 * len(part["x"]))
+    g.set_aux(True)
+    o = oracle.OracleSim(prm, part, d["Z"])
+    o.step()
+    g.step(1)
+    for k in ("x", "y", "vpx", "vpy", "hp", "F_11", "F_12", "F_22", "Ap", "Fb_x", "Fb_y"):
+        assert np.array_equal(bits(g.get(k)), bits(o.get(k))), k
+    for k in ("Mv", "vvxL", "vvyL"):
+        assert np.array_equal(bits(g.get(k)), bits(o.get(k))), k
+    del o
+    g.set_aux(False)
+    m0 = float(np.sum(d["Mp"]))
+    worst = 0.0
+    ms_steps = 0
+    for it in range(2, 23):
+        assert g.step(1) == 1
+        em, eo, _ = g.conservation()
+        worst = max(worst, em, eo)
+        ms_steps += int(prm["ms_on"] and it % prm["ms_interval"] == 0)
+    assert worst < 1e-10
+    assert ms_steps >= 1
+    Mp = g.get("Mp")
+    assert abs(Mp.sum() - m0) <= 1e-12 * m0
+    assert np.isfinite(g.get("x")).all() and np.isfinite(g.get("hp")).all()
+    g.close()
+
+
+def test_lake_at_rest_over_the_4m_topography():
+    from amsc_amd.simulation import Simulation
+    print("[full size] lake at rest, 2048x2048", flush=True)
+    d = scenarios.large_fractal(still_water=True, nsteps=200)
+    prm = params_from_data(d)
+    part = particles_from_data(d)
+    g = Simulation(prm, part, d["Z"])
+    g.step(200)
+    nex = int(d["Nex"])
+    hx, hy = prm["hx"], prm["hy"]
+    Zg = np.asarray(d["Z"], np.float64).reshape(nex + 1, nex + 1)
+    x, y, hp = g.get("x"), g.get("y"), g.get("hp")
+    c = np.minimum(np.floor(x / hx).astype(np.int64), nex - 1)
+    r = np.minimum(np.floor(y / hy).astype(np.int64), nex - 1)
+    xi, eta = (x - c * hx) / hx, (y - r * hy) / hy
+    zp = ((1 - xi) * (1 - eta) * Zg[c, r] + (1 - xi) * eta * Zg[c, r + 1]
+          + xi * (1 - eta) * Zg[c + 1, r] + xi * eta * Zg[c + 1, r + 1])
+    assert np.abs(hp + zp - 10.0).max() < 1e-10
+    assert np.abs(x - part["x"]).max() < 1e-10 and np.abs(y - part["y"]).max() < 1e-10
+    assert np.hypot(g.get("vpx"), g.get("vpy")).max() < 1e-10
+    g.close()
