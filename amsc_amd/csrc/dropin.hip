@@ -23,12 +23,36 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Colour of each cell, rebuilt from (ccidx, coloff) on every scatter call.
-struct ColourMap {
-  uint8_t *d = nullptr;
+// Device memory kept between calls: grown when a call needs more, and
+// allocated again when the caller's current device has changed.  hipFree
+// waits for the device, so a buffer is never freed under a kernel that an
+// earlier, still running call launched on it.
+struct DevCache {
+  int dev = -1;
+  void *p = nullptr;
   size_t cap = 0;
 };
-ColourMap g_colour;
+
+int dev_cache_get(DevCache &c, size_t bytes, void **out, const char *what) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return fail(AMSC_ERR_HIP, std::string(what) + ": no device");
+  if (dev != c.dev || bytes > c.cap) {
+    if (c.p) (void)hipFree(c.p);
+    c = DevCache();
+    if (hipMalloc(&c.p, bytes) != hipSuccess) {
+      c.p = nullptr;
+      return fail(AMSC_ERR_NOMEM, std::string(what) + ": workspace allocation");
+    }
+    c.dev = dev;
+    c.cap = bytes;
+  }
+  *out = c.p;
+  return AMSC_OK;
+}
+
+// Colour of each cell, rebuilt from (ccidx, coloff) on every scatter call.
+DevCache g_colour;
 std::mutex g_colour_mu;
 
 __global__ void fill_u8_kernel(uint8_t *p, int n, uint8_t v) {
@@ -65,20 +89,16 @@ int prepare_colouring(int nrows, const int *ccidx, const int *coloff,
     return AMSC_OK;
   }
   std::lock_guard<std::mutex> lk(g_colour_mu);
-  if (g_colour.cap < (size_t)nc) {
-    if (g_colour.d) (void)hipFree(g_colour.d);
-    g_colour.d = nullptr;
-    g_colour.cap = 0;
-    if (hipMalloc(&g_colour.d, (size_t)nc) != hipSuccess)
-      return fail(AMSC_ERR_NOMEM, "colour map allocation failed");
-    g_colour.cap = nc;
-  }
+  void *p = nullptr;
+  if (int rc = dev_cache_get(g_colour, (size_t)nc, &p, "colour map"))
+    return rc;
+  uint8_t *map = static_cast<uint8_t *>(p);
   hipStream_t st = current_stream();
-  fill_u8_kernel<<<grid_for(nc, kBlock), kBlock, 0, st>>>(g_colour.d, nc, 0xFF);
+  fill_u8_kernel<<<grid_for(nc, kBlock), kBlock, 0, st>>>(map, nc, 0xFF);
   colour_map_kernel<<<grid_for(nc, kBlock), kBlock, 0, st>>>(
-      ccidx, nc, coloff[1], coloff[2], coloff[3], g_colour.d);
+      ccidx, nc, coloff[1], coloff[2], coloff[3], map);
   AMSC_LAUNCH_CHECK();
-  *colour = g_colour.d;
+  *colour = map;
   return AMSC_OK;
 }
 
@@ -263,7 +283,7 @@ int tiled_gather(int nrows, double hx, double hy, const int *cstart,
 // ---- CFL maxima (gpu_offload.cpp:285-292): max |vpx|, max |vpy|, max hp.
 // Each workgroup writes its three maxima, one workgroup reduces them (max is
 // exact in any order).
-constexpr int kCflBlock = 256;
+constexpr int kCflBlock = 256, kCflMaxBlocks = 1024;
 
 __device__ __forceinline__ double wave_max_d(double v) {
 #pragma unroll
@@ -862,10 +882,17 @@ int amsc_cfl_dt(int np, const double *d_vpx, const double *d_vpy,
                 int it, double *dt, double *cel_out) {
   if (np <= 0 || !d_vpx || !d_vpy || !d_hp || !dt)
     return fail(AMSC_ERR_INVALID, "amsc_cfl_dt: bad arguments");
-  const int nb = std::min((int)grid_for(np, kCflBlock), 1024);
-  double *part = nullptr;
-  if (hipMalloc(&part, sizeof(double) * (3 * nb + 3)) != hipSuccess)
-    return fail(AMSC_ERR_NOMEM, "amsc_cfl_dt: workspace allocation");
+  const int nb = std::min((int)grid_for(np, kCflBlock), kCflMaxBlocks);
+  // one workspace of the largest size, held (with the lock) until the
+  // read-back below has completed
+  static DevCache cache;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  void *p = nullptr;
+  if (int rc = dev_cache_get(cache, sizeof(double) * (3 * kCflMaxBlocks + 3),
+                             &p, "amsc_cfl_dt"))
+    return rc;
+  double *part = static_cast<double *>(p);
   hipStream_t st = current_stream();
   cfl_partial_kernel<<<nb, kCflBlock, 0, st>>>(np, d_vpx, d_vpy, d_hp, part);
   cfl_final_kernel<<<1, kCflBlock, 0, st>>>(nb, part, part + 3 * nb);
@@ -876,7 +903,6 @@ int amsc_cfl_dt(int np, const double *d_vpx, const double *d_vpy,
           hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     rc = fail(AMSC_ERR_HIP, "amsc_cfl_dt: reduction failed");
-  (void)hipFree(part);
   if (rc) return rc;
   // gpu_offload.cpp:293-298
   const double max_vx = m[0], max_vy = m[1], hmax = m[2];
