@@ -364,6 +364,15 @@ struct P2GArgs {
   const P2GOut *out;
 };
 
+// A pointer read from device memory is generic to the compiler, and a store
+// through it is a FLAT store, which the LDS wait counter also counts: a FLAT
+// store that may still be in flight forces every later LDS read to wait for
+// all LDS reads before it.  These pointers always address global memory.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *global_ptr(void *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 template <typename R> struct P2GRaw {
   double x, y, hp, H;
   R Mp, m1, m2, Vp, frx, fry, f11, f12, f22;
@@ -599,12 +608,12 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
           ppy += my;
           if (a.aux >= AUX_FULL) {
             const P2GOut *o = a.out;
-            static_cast<R *>(o->dzx)[I1.ip] = (R)dzx;
-            static_cast<R *>(o->dzy)[I1.ip] = (R)dzy;
-            static_cast<R *>(o->fpx)[I1.ip] = (R)Fpx;
-            static_cast<R *>(o->fpy)[I1.ip] = (R)Fpy;
-            static_cast<R *>(o->momx)[I1.ip] = (R)mx;
-            static_cast<R *>(o->momy)[I1.ip] = (R)my;
+            global_ptr<R>(o->dzx)[I1.ip] = (R)dzx;
+            global_ptr<R>(o->dzy)[I1.ip] = (R)dzy;
+            global_ptr<R>(o->fpx)[I1.ip] = (R)Fpx;
+            global_ptr<R>(o->fpy)[I1.ip] = (R)Fpy;
+            global_ptr<R>(o->momx)[I1.ip] = (R)mx;
+            global_ptr<R>(o->momy)[I1.ip] = (R)my;
           }
         }
       }
