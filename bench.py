@@ -530,7 +530,14 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "avg_launch_ms": round(r64["p2g_ms"], 5),
-                     "algorithmic_bytes_per_launch": int(algo_bytes)},
+                     "algorithmic_bytes_per_launch": int(algo_bytes),
+                     # the same kernel's launches inside time steps (events
+                     # attached to each dispatch, which add a few us each;
+                     # the particle data is no longer in the caches there)
+                     "in_step_avg_launch_ms": r64["kernels"].get("p2g_nodal"),
+                     "in_step_frac": (round(algo_bytes / (r64["kernels"]["p2g_nodal"] * 1e-3)
+                                            / 1e9 / HBM_PEAK_GBS, 4)
+                                      if r64["kernels"].get("p2g_nodal") else None)},
         "step_roofline": {"algorithmic_bytes_per_step": int(step_bytes),
                           "achieved": round(step_gbs, 1),
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
