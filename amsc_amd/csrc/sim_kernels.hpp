@@ -679,23 +679,23 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     static_cast<R *>(a.vvyL)[n] = (R)vyL;
     if (a.aux >= AUX_FULL) {
       double *const *nod = a.out->nod;
-      nod[NF_AVX][n] = ax;
-      nod[NF_AVY][n] = ay;
-      nod[NF_MV][n] = Mv;
-      nod[NF_MOMX][n] = mx;
-      nod[NF_MOMY][n] = my;
-      nod[NF_FEXTX][n] = Fex;
-      nod[NF_FEXTY][n] = Fey;
-      nod[NF_FINTX][n] = Fix;
-      nod[NF_FINTY][n] = Fiy;
-      nod[NF_FRX][n] = Frx;
-      nod[NF_FRY][n] = Fry;
-      nod[NF_FPX][n] = FPx;
-      nod[NF_FPY][n] = FPy;
-      nod[NF_FTOTX][n] = Ftx;
-      nod[NF_FTOTY][n] = Fty;
-      nod[NF_VVX][n] = vx;
-      nod[NF_VVY][n] = vy;
+      global_ptr<double>(nod[NF_AVX])[n] = ax;
+      global_ptr<double>(nod[NF_AVY])[n] = ay;
+      global_ptr<double>(nod[NF_MV])[n] = Mv;
+      global_ptr<double>(nod[NF_MOMX])[n] = mx;
+      global_ptr<double>(nod[NF_MOMY])[n] = my;
+      global_ptr<double>(nod[NF_FEXTX])[n] = Fex;
+      global_ptr<double>(nod[NF_FEXTY])[n] = Fey;
+      global_ptr<double>(nod[NF_FINTX])[n] = Fix;
+      global_ptr<double>(nod[NF_FINTY])[n] = Fiy;
+      global_ptr<double>(nod[NF_FRX])[n] = Frx;
+      global_ptr<double>(nod[NF_FRY])[n] = Fry;
+      global_ptr<double>(nod[NF_FPX])[n] = FPx;
+      global_ptr<double>(nod[NF_FPY])[n] = FPy;
+      global_ptr<double>(nod[NF_FTOTX])[n] = Ftx;
+      global_ptr<double>(nod[NF_FTOTY])[n] = Fty;
+      global_ptr<double>(nod[NF_VVX])[n] = vx;
+      global_ptr<double>(nod[NF_VVY])[n] = vy;
     }
   }
 
