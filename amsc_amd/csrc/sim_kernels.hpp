@@ -789,7 +789,16 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
     gy[k] = static_cast<const R *>(a.vvyL)[nid[k]];
   }
   const double x = fld<double>(p, PF_X)[ip], y = fld<double>(p, PF_Y)[ip];
-  const Local L0 = local_coords(x, y, c, r, hx, hy, sc.rc);
+  // Every other per-particle load is issued here, with the first ones: a
+  // load first written at its use sits behind the branches before it and
+  // costs a memory round trip of its own.  The reciprocals are copied for
+  // the same reason (read through the reference, they were loaded again in
+  // each branch that uses them).
+  const double hp0 = fld<double>(p, PF_HP)[ip];
+  const A Vp = fld<R>(p, PF_VP)[ip];
+  const double H = fld<double>(p, PF_H)[ip];
+  const Recip rc = sc.rc;
+  const Local L0 = local_coords(x, y, c, r, hx, hy, rc);
   // g2p(vvxL, vvyL), optim_par.cpp:542
   const A xi0 = (A)L0.xi, eta0 = (A)L0.eta;
   A vxL = 0, vyL = 0;
@@ -809,7 +818,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   if (yn > Ly - eps) yn = Ly - eps;
   // g2pd(vvxL, vvyL) at the new position, step-start cell (D18),
   // optim_par.cpp:578
-  const Local L1 = local_coords(xn, yn, c, r, hx, hy, sc.rc);
+  const Local L1 = local_coords(xn, yn, c, r, hx, hy, rc);
   const A xi1 = (A)L1.xi, eta1 = (A)L1.eta;
   const RecipT<A> rcr = recip_of<A>(sc);
   A vxdx = 0, vxdy = 0, vydx = 0, vydy = 0;
@@ -824,9 +833,8 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   }
   // height update with floor, area, optim_par.cpp:595-607 (FP64)
   const A div = vxdx + vydy;
-  double h = fld<double>(p, PF_HP)[ip] / (1 + dt * (double)div);
+  double h = hp0 / (1 + dt * (double)div);
   h = h < 1e-2 ? 1e-2 : h;
-  const A Vp = fld<R>(p, PF_VP)[ip];
   const A hR = (A)h;
   A Fbx, Fby, Ap;
   // from the velocity as stored, which is what a P2G that recomputes the
@@ -864,7 +872,6 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
     const A sig_xy = invII != A(0) ? (tau_Y / sqrt(invII) + A(2) * mu) * D_xy : A(0);
     const A sig_yy = invII != A(0) ? (tau_Y / sqrt(invII) + A(2) * mu) * D_yy : A(0);
     // hydrostatic part with the well-balance correction, FP64
-    const double H = fld<double>(p, PF_H)[ip];
     const double h_corr = h > 1e-10 ? h - (H * H) / h : 0.0;
     if (sizeof(A) == sizeof(double)) {
       F11 = P.bing * sig_xx - .5 * P.rho * P.g * h_corr;
@@ -917,7 +924,7 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
     fld<R>(p, PF_MOMX)[ip] = vpx * Mp;
     fld<R>(p, PF_MOMY)[ip] = vpy * Mp;
   }
-  newkey = sc.coff + cell_of(xn, yn, P.nrows, P.ncols, hx, hy, sc.rc);
+  newkey = sc.coff + cell_of(xn, yn, P.nrows, P.ncols, hx, hy, rc);
   vmx = fabs((double)vpx);
   vmy = fabs((double)vpy);
   hmx = h > 0.0 ? h : 0.0;
@@ -957,7 +964,9 @@ __global__ void __launch_bounds__(BS)
   if (uniform == EDGE) return;
   const int ip = b0 + threadIdx.x;
   const bool valid = ip < np;
-  int sid = sb;
+  // one scenario per workgroup in the main launch: say so to the compiler,
+  // so that the scenario's parameters are read with scalar loads
+  int sid = EDGE ? sb : __builtin_amdgcn_readfirstlane(sb);
   if (EDGE && valid)
     while (sid + 1 < T.S && ip >= T.poff[sid + 1]) ++sid;
   const Ctrl &C = T.ctrl[sid];
