@@ -432,8 +432,14 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     const int xcd = b & 7, q8 = nb >> 3, rem = nb & 7;
     tile = xcd * q8 + (xcd < rem ? xcd : rem) + (b >> 3);
   }
-  const int sid = a.tscen[tile];
+  // workgroup-uniform, said explicitly
+  const int sid = __builtin_amdgcn_readfirstlane(a.tscen[tile]);
   const Scen &sc = a.T.scen[sid];
+  // the parameters the staging reads, loaded here, in uniform control flow
+  // (scalar loads): read through sc inside the friction branches they were
+  // per-lane loads, and each one's wait also waited for the next chunk's
+  // prefetched fields
+  const KParams Pv = sc.P;
   Ctrl *ctrl = &a.T.ctrl[sid];
   if (!ctrl->active) return;
   // stored friction force (first step, after merge/split or a field
@@ -591,7 +597,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
         R frx = W.frx, fry = W.fry;
         if (!fric_read) {
           A fbx, fby, ap;
-          voellmy<A>(sc.P, (A)W.m1, (A)W.m2, W.hp, (A)W.Vp, fbx, fby, ap);
+          voellmy<A>(Pv, (A)W.m1, (A)W.m2, W.hp, (A)W.Vp, fbx, fby, ap);
           frx = ap * fbx;
           fry = ap * fby;
         }
