@@ -81,6 +81,28 @@ __device__ __forceinline__ double shape(const Local &L, int inode) {
   }
 }
 
+// shape, shape_dx and shape_dy for a corner known only at run time (a
+// node's cells in a caller's colour order), without branches: every case
+// is one product of two factors, each chosen by one bit of inode, so the
+// values are the switch's bit for bit.
+__device__ __forceinline__ double shape_sel(const Local &L, int inode) {
+  const double a = (inode & 2) ? L.xi : (1.0 - L.xi);
+  const double b = (inode & 1) ? L.eta : (1.0 - L.eta);
+  return a * b;
+}
+__device__ __forceinline__ double shape_dx_sel(const Local &L, int inode,
+                                               double hx) {
+  const double f = (inode & 2) ? (1.0 / hx) : (-1.0 / hx);
+  const double b = (inode & 1) ? L.eta : (1.0 - L.eta);
+  return f * b;
+}
+__device__ __forceinline__ double shape_dy_sel(const Local &L, int inode,
+                                               double hy) {
+  const double a = (inode & 2) ? L.xi : (1.0 - L.xi);
+  const double f = (inode & 1) ? (1.0 / hy) : (-1.0 / hy);
+  return a * f;
+}
+
 // dN_inode/dx (gpu_kernels.cpp:43-59, idir 0)
 __device__ __forceinline__ double shape_dx(const Local &L, int inode,
                                            double hx) {

@@ -243,13 +243,13 @@ __global__ void __launch_bounds__(kTNT)
           Lq.xi = s_xy[0][i];
           Lq.eta = s_xy[1][i];
           if (INTERNAL) {
-            const double Nx = shape_dx(Lq, cn[k], hx);
-            const double Ny = shape_dy(Lq, cn[k], hy);
+            const double Nx = shape_dx_sel(Lq, cn[k], hx);
+            const double Ny = shape_dy_sel(Lq, cn[k], hy);
             const double vp = s_in[0][i];
             acc[0] += (Nx * s_in[1][i] + Ny * s_in[3][i]) * vp;
             acc[NOUT - 1] += (Nx * s_in[2][i] + Ny * s_in[4][i]) * vp;
           } else {
-            const double N = shape(Lq, cn[k]);
+            const double N = shape_sel(Lq, cn[k]);
 #pragma unroll
             for (int o = 0; o < NOUT; ++o) acc[o] += N * s_in[o][i];
           }
