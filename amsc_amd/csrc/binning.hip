@@ -402,3 +402,111 @@ extern "C" int amsc_bin_particles(int np, int nrows, int ncols, double hx,
   if (rc) bin_cache_drop(c);  // the counts may not be back at zero
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// amsc_bin_particles_noalloc: the same binning with its workspace (cell
+// counts, arrival slots, scan tile sums) in a fixed array of the library's
+// own device memory, for callers that cannot allocate (code under graph
+// capture, serving loops).  Each GPU holds its own copy of the array from
+// the library's load to the end of the process, at an address that
+// hipGetSymbolAddress gives with that device current; the array's name in
+// host code is a host-side stand-in and is never handed to a kernel.
+namespace amsc {
+
+constexpr int kNoAllocMaxCells = 1 << 22;      // a 2048 x 2048 grid
+constexpr int kNoAllocMaxParticles = 1 << 22;  // 4,194,304 particles
+constexpr int kNoAllocScanInts =
+    ((kNoAllocMaxCells + kScanTile - 1) / kScanTile + 63) / 64 * 64;
+constexpr size_t kNoAllocInts = (size_t)kNoAllocMaxCells +
+                                kNoAllocMaxParticles + kNoAllocScanInts;
+
+// counts [0, kNoAllocMaxCells), slots [.., + kNoAllocMaxParticles), scan
+// tile sums after them: 32 MiB + 4 KiB per GPU
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+std::mutex &noalloc_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+// Device address of this GPU's scratch, looked up once per device (the
+// caller holds noalloc_mutex()).
+int noalloc_scratch(int **out) {
+  static int *ptr[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(AMSC_ERR_HIP, "amsc_bin_particles_noalloc: no device");
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(AMSC_ERR_HIP, std::string("amsc_bin_particles_noalloc: "
+                                            "scratch address: ") +
+                                    hipGetErrorString(e));
+    ptr[dev] = static_cast<int *>(p);
+  }
+  *out = ptr[dev];
+  return AMSC_OK;
+}
+
+__global__ void clear_ints_kernel(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace
+}  // namespace amsc
+
+extern "C" int amsc_bin_particles_noalloc(int np, int nrows, int ncols,
+                                          double hx, double hy,
+                                          const double *d_x, const double *d_y,
+                                          int *d_p2g, int *d_cstart,
+                                          int *d_cptcls, int *d_ccidx,
+                                          int *coloff) {
+  if (np < 0 || nrows <= 0 || ncols <= 0)
+    return fail(AMSC_ERR_INVALID, "amsc_bin_particles_noalloc: bad sizes");
+  if ((long long)nrows * ncols > kNoAllocMaxCells)
+    return fail(AMSC_ERR_INVALID,
+                "amsc_bin_particles_noalloc: more than 4194304 cells (the "
+                "size of its fixed workspace); use amsc_bin_particles");
+  if (np > kNoAllocMaxParticles)
+    return fail(AMSC_ERR_INVALID,
+                "amsc_bin_particles_noalloc: more than 4194304 particles (the "
+                "size of its fixed workspace); use amsc_bin_particles");
+  if (!d_p2g || !d_cstart || (np > 0 && !d_cptcls))
+    return fail(AMSC_ERR_INVALID, "amsc_bin_particles_noalloc: NULL output");
+  const int nc = nrows * ncols;
+  hipStream_t st = current_stream();
+  // one call at a time per process: every call on a GPU shares its scratch
+  std::lock_guard<std::mutex> lock(noalloc_mutex());
+  int *ws = nullptr;
+  if (int rc = noalloc_scratch(&ws)) return rc;
+  int *count = ws, *slot = ws + kNoAllocMaxCells;
+  // the counts start at zero (the scratch keeps whatever the last call
+  // left, which after a failed call need not be zero)
+  clear_ints_kernel<<<grid_for(nc, 256), 256, 0, st>>>(count, nc);
+  Binning b;
+  b.nrows = nrows;
+  b.ncols = ncols;
+  b.count = count;
+  b.cstart = d_cstart;
+  b.scan.tmp = ws + kNoAllocMaxCells + kNoAllocMaxParticles;
+  b.scan.bytes = sizeof(int) * kNoAllocScanInts;
+  b.scan.owned = false;  // never reallocated: a scan that needs more fails
+  int rc = AMSC_OK;
+  if (hipGetLastError() != hipSuccess)
+    rc = fail(AMSC_ERR_HIP, "amsc_bin_particles_noalloc: clear failed");
+  if (!rc)
+    rc = launch_key_count(np, d_x, d_y, nrows, ncols, hx, hy, d_p2g, slot,
+                          count, nullptr, st);
+  if (!rc) rc = launch_finish_binning(b, np, d_p2g, slot, d_cptcls, nullptr, st);
+  if (!rc && coloff) rc = launch_colouring(nrows, ncols, d_ccidx, coloff, st);
+  // complete before returning, like amsc_bin_particles
+  const hipError_t e = hipStreamSynchronize(st);
+  if (!rc && e != hipSuccess) rc = fail(AMSC_ERR_HIP, hipGetErrorString(e));
+  return rc;
+}

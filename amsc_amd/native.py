@@ -118,6 +118,9 @@ def lib():
     L.amsc_bin_particles.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
                                      C.c_double, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _ip]
+    L.amsc_bin_particles_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _ip]
     L.amsc_ms_decide.argtypes = [C.POINTER(AmscParams), C.c_int, C.c_double] + \
         [_vp] * 15 + [_ip]
     L.amsc_cfl_dt.argtypes = [C.c_int, _vp, _vp, _vp, C.c_double, C.c_double,
@@ -199,10 +202,13 @@ def dropin(name):
 
 
 def bin_particles(np_, nrows, ncols, hx, hy, d_x, d_y, d_p2g, d_cstart,
-                  d_cptcls, d_ccidx=None):
+                  d_cptcls, d_ccidx=None, noalloc=False):
+    """amsc_bin_particles (noalloc: amsc_bin_particles_noalloc); returns
+    coloff."""
     coloff = np.zeros(5, np.int32)
-    check(lib().amsc_bin_particles(np_, nrows, ncols, hx, hy, d_x, d_y, d_p2g,
-                                   d_cstart, d_cptcls, d_ccidx, coloff))
+    fn = lib().amsc_bin_particles_noalloc if noalloc else lib().amsc_bin_particles
+    check(fn(np_, nrows, ncols, hx, hy, d_x, d_y, d_p2g, d_cstart, d_cptcls,
+             d_ccidx, coloff))
     return coloff
 
 

@@ -217,6 +217,23 @@ int amsc_bin_particles(int np, int nrows, int ncols, double hx, double hy,
                        int *d_cstart, int *d_cptcls, int *d_ccidx,
                        int *coloff);
 
+/* No counterpart in the reference: amsc_bin_particles for callers that
+ * cannot allocate device memory (code under graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and
+ * completion rule; it calls no hipMalloc, hipFree or allocating library
+ * routine.  Its workspace (cell counts, arrival slots, scan tile sums) is a
+ * fixed array in the library's own device memory, 32 MiB per GPU, held from
+ * the library's load to the end of the process: at most 4194304 cells
+ * (nrows*ncols) and 4194304 particles, larger sizes are rejected with
+ * AMSC_ERR_INVALID before any device work.  Every call on one GPU shares
+ * that GPU's array; calls from one process are serialised, calls that
+ * overlap on one GPU from several threads of control outside that rule
+ * (e.g. two processes sharing a context) would race. */
+int amsc_bin_particles_noalloc(int np, int nrows, int ncols, double hx,
+                               double hy, const double *d_x, const double *d_y,
+                               int *d_p2g, int *d_cstart, int *d_cptcls,
+                               int *d_ccidx, int *coloff);
+
 /* Solver parameters: the DATA.json scalars (main/mpm_data.h:14-56), names in
  * lower case; nrows = Ney, ncols = Nex; phi in degrees; max_steps > 0 caps
  * an adaptive run; ms_* as DATA's merge/split keys. */
