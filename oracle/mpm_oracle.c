@@ -1166,41 +1166,75 @@ void or_elfs_cells(int nrows, int ncols, double h, const int *cstart,
 }
 
 /* decide_actions, merge_split_ops.h:349-510 (flags :388-402, then the
- * per-cell pass with a 1 -> 4 split and at least 2 particles per cell). */
-static void elfs_decide(or_sim *s, int *act, int *partner, int counts[2]) {
-  const or_params *p = &s->p;
-  const int np = s->np, nc = s->nc;
+ * per-cell pass with a 1 -> 4 split and at least 2 particles per cell), on
+ * arrays binned by p2g / cstart / cptcls. */
+static void elfs_decide_core(const or_params *p, int np, const double *x,
+                             const double *y, const double *Ap,
+                             const double *hp, const double *vpx_dx,
+                             const double *vpy_dy, const double *vpx,
+                             const double *vpy, const int *level,
+                             const int *p2g, const int *cstart,
+                             const int *cptcls, int *act, int *partner,
+                             int counts[2]) {
+  const int nc = p->nrows * p->ncols;
   char *ext = (char *)malloc(nc > 0 ? nc : 1);
   double *dist = (double *)malloc(sizeof(double) * (nc > 0 ? nc : 1));
   int *q1 = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
   int *q2 = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
-  elfs_exterior(p->nrows, p->ncols, s->cstart, ext, q1);
-  elfs_distance(p->nrows, p->ncols, p->hx < p->hy ? p->hx : p->hy, s->cstart,
+  elfs_exterior(p->nrows, p->ncols, cstart, ext, q1);
+  elfs_distance(p->nrows, p->ncols, p->hx < p->hy ? p->hx : p->hy, cstart,
                 ext, dist, q1, q2);
-  double **f = s->f;
   for (int ip = 0; ip < np; ++ip) {
-    const double e = elfs_one(p, s->p2g[ip], f[OR_X][ip], f[OR_Y][ip], dist, ext);
-    const double r_i = sqrt(f[OR_AP][ip]);
-    const double divergence = f[OR_VPX_DX][ip] + f[OR_VPY_DY][ip];
+    const double e = elfs_one(p, p2g[ip], x[ip], y[ip], dist, ext);
+    const double r_i = sqrt(Ap[ip]);
+    const double divergence = vpx_dx[ip] + vpy_dy[ip];
     act[ip] = 0;
     if ((e < p->ms_alpha * r_i) || (divergence > p->ms_stretch)) {
-      if (s->level[ip] > p->ms_min_level) act[ip] = 1;
-    } else if (e > p->ms_beta * r_i && f[OR_HP][ip] > p->ms_hp_min) {
-      if (s->level[ip] < p->ms_max_level) act[ip] = 2;
+      if (level[ip] > p->ms_min_level) act[ip] = 1;
+    } else if (e > p->ms_beta * r_i && hp[ip] > p->ms_hp_min) {
+      if (level[ip] < p->ms_max_level) act[ip] = 2;
     }
   }
-  ms_pair_csr(p, np, nc, f[OR_X], f[OR_Y], f[OR_VPX], f[OR_VPY], s->cstart,
-              s->cptcls, act, partner, counts, 4, 2);
+  ms_pair_csr(p, np, nc, x, y, vpx, vpy, cstart, cptcls, act, partner, counts,
+              4, 2);
   free(ext); free(dist); free(q1); free(q2);
+}
+
+static void elfs_decide(or_sim *s, int *act, int *partner, int counts[2]) {
+  double **f = s->f;
+  elfs_decide_core(&s->p, s->np, f[OR_X], f[OR_Y], f[OR_AP], f[OR_HP],
+                   f[OR_VPX_DX], f[OR_VPY_DY], f[OR_VPX], f[OR_VPY], s->level,
+                   s->p2g, s->cstart, s->cptcls, act, partner, counts);
+}
+
+void or_elfs_decide(const or_params *p, int np, const double *x,
+                    const double *y, const double *Ap, const double *hp,
+                    const double *vpx_dx, const double *vpy_dy,
+                    const double *vpx, const double *vpy, const int *level,
+                    int *action, int *partner, int counts[2]) {
+  const int nc = p->nrows * p->ncols;
+  int *p2g = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+  int *cptcls = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+  int *cstart = (int *)malloc(sizeof(int) * (nc + 1));
+  int *ccidx = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
+  int coloff[5];
+  or_bin_particles(np, x, y, p->nrows, p->ncols, p->hx, p->hy, p2g, cstart,
+                   cptcls, ccidx, coloff);
+  elfs_decide_core(p, np, x, y, Ap, hp, vpx_dx, vpy_dy, vpx, vpy, level, p2g,
+                   cstart, cptcls, action, partner, counts);
+  free(p2g); free(cptcls); free(cstart); free(ccidx);
 }
 
 /* execute_merge_split, merge_split_ops.h:513-716: weights {1,4,1,0}; a
  * split leaves four children at (x +- r/2, y +- r/2), r = sqrt(Ap), clamped
  * to the domain, extensive fields quartered, level - 1, label -1; a merge
  * averages the intensive fields by mass, sums the extensive ones, keeps
- * the primary's label, level + 1; momentum = Mp * v for both. */
-static void elfs_execute(or_sim *s, const int *act, const int *partner) {
-  const int N = s->np;
+ * the primary's label, level + 1; momentum = Mp * v for both.  Arrays as
+ * ms_execute_core; returns the new count. */
+static int elfs_execute_core(const or_params *p, int N, const int *act,
+                             const int *partner, double *const *f,
+                             const int *label, const int *level, double **nf,
+                             int *nlabel, int *nlevel) {
   static const int weights[4] = {1, 4, 1, 0};
   int *offset = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
   int Nnew = 0;
@@ -1208,20 +1242,14 @@ static void elfs_execute(or_sim *s, const int *act, const int *partner) {
     offset[i] = Nnew;
     Nnew += weights[act[i]];
   }
-  double *nf[OR_NFIELDS];
-  for (int k = 0; k < OR_NFIELDS; ++k)
-    nf[k] = (double *)malloc(sizeof(double) * (Nnew > 0 ? Nnew : 1));
-  int *nlabel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
-  int *nlevel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
-  const double Lx = s->p.ncols * s->p.hx, Ly = s->p.nrows * s->p.hy;
-  double **f = s->f;
+  const double Lx = p->ncols * p->hx, Ly = p->nrows * p->hy;
   for (int i = 0; i < N; ++i) {
     const int a = act[i];
     if (a == 0) {
       const int o = offset[i];
       for (int k = 0; k < OR_NFIELDS; ++k) nf[k][o] = f[k][i];
-      nlabel[o] = s->label[i];
-      nlevel[o] = s->level[i];
+      nlabel[o] = label[i];
+      nlevel[o] = level[i];
     } else if (a == 1) {
       const int o = offset[i];
       const double px = f[OR_X][i], py = f[OR_Y][i];
@@ -1249,7 +1277,7 @@ static void elfs_execute(or_sim *s, const int *act, const int *partner) {
         nf[OR_YP][o + c] = cy[c];
         nf[OR_MOM_PX][o + c] = nf[OR_MP][o + c] * nf[OR_VPX][o + c];
         nf[OR_MOM_PY][o + c] = nf[OR_MP][o + c] * nf[OR_VPY][o + c];
-        nlevel[o + c] = s->level[i] - 1;
+        nlevel[o + c] = level[i] - 1;
         nlabel[o + c] = -1;
       }
     } else if (a == 2) {
@@ -1266,10 +1294,26 @@ static void elfs_execute(or_sim *s, const int *act, const int *partner) {
       nf[OR_XP][o] = nx; nf[OR_YP][o] = ny;
       nf[OR_MOM_PX][o] = nf[OR_MP][o] * nf[OR_VPX][o];
       nf[OR_MOM_PY][o] = nf[OR_MP][o] * nf[OR_VPY][o];
-      nlabel[o] = s->label[i];
-      nlevel[o] = s->level[i] + 1;
+      nlabel[o] = label[i];
+      nlevel[o] = level[i] + 1;
     }
   }
+  free(offset);
+  return Nnew;
+}
+
+static void elfs_execute(or_sim *s, const int *act, const int *partner) {
+  static const int weights[4] = {1, 4, 1, 0};
+  const int N = s->np;
+  int Nnew = 0;
+  for (int i = 0; i < N; ++i) Nnew += weights[act[i]];
+  double *nf[OR_NFIELDS];
+  for (int k = 0; k < OR_NFIELDS; ++k)
+    nf[k] = (double *)malloc(sizeof(double) * (Nnew > 0 ? Nnew : 1));
+  int *nlabel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
+  int *nlevel = (int *)malloc(sizeof(int) * (Nnew > 0 ? Nnew : 1));
+  elfs_execute_core(&s->p, N, act, partner, s->f, s->label, s->level, nf,
+                    nlabel, nlevel);
   sim_reserve(s, Nnew > 0 ? Nnew : 1);
   for (int k = 0; k < OR_NFIELDS; ++k) {
     memcpy(s->f[k], nf[k], sizeof(double) * Nnew);
@@ -1277,8 +1321,15 @@ static void elfs_execute(or_sim *s, const int *act, const int *partner) {
   }
   memcpy(s->label, nlabel, sizeof(int) * Nnew);
   memcpy(s->level, nlevel, sizeof(int) * Nnew);
-  free(nlabel); free(nlevel); free(offset);
+  free(nlabel); free(nlevel);
   s->np = Nnew;
+}
+
+int or_elfs_execute(const or_params *p, int np, const int *act,
+                    const int *partner, double *const *in, const int *label,
+                    const int *level, double **out, int *nlabel, int *nlevel) {
+  return elfs_execute_core(p, np, act, partner, in, label, level, out, nlabel,
+                           nlevel);
 }
 
 /* stress_tensor_t::operator(), optim_par.cpp:68-120 (Bingham, with the

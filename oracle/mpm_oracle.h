@@ -215,6 +215,18 @@ int or_ms_execute(const or_params *p, int np, const int *act,
                   const int *partner, double *const *in, const int *label,
                   const int *level, double **out, int *nlabel, int *nlevel);
 int or_field_count(void);
+/* The legacy ELFS rule (ms_mode 1) on a snapshot: decide_actions of
+ * merge_split_ops.h:349-510 with the cells of or_bin_particles(x, y) and
+ * execute_merge_split of merge_split_ops.h:513-716 (arrays as
+ * or_ms_execute, room for np + 3 * splits). */
+void or_elfs_decide(const or_params *p, int np, const double *x,
+                    const double *y, const double *Ap, const double *hp,
+                    const double *vpx_dx, const double *vpy_dy,
+                    const double *vpx, const double *vpy, const int *level,
+                    int *action, int *partner, int counts[2]);
+int or_elfs_execute(const or_params *p, int np, const int *act,
+                    const int *partner, double *const *in, const int *label,
+                    const int *level, double **out, int *nlabel, int *nlevel);
 
 #ifdef __cplusplus
 }

@@ -153,14 +153,16 @@ def field_names():
     return [L.or_field_name(k).decode() for k in range(L.or_field_count())]
 
 
-def ms_execute(prm: dict, act, partner, fields: dict, label, level):
-    """execute_merge_split (merge_split_ops_cmes.h:258-427) on a snapshot:
-    fields maps every name of field_names() to an array; returns (fields,
-    label, level) of the new particle set."""
+def ms_execute(prm: dict, act, partner, fields: dict, label, level, elfs=False):
+    """execute_merge_split (merge_split_ops_cmes.h:258-427; elfs: the legacy
+    rule's, merge_split_ops.h:513-716) on a snapshot: fields maps every name
+    of field_names() to an array; returns (fields, label, level) of the new
+    particle set."""
     names = field_names()
     act = np.ascontiguousarray(act, np.int32)
     n = len(act)
-    nnew = int(np.sum(np.array([1, 2, 1, 0])[act])) if n else 0
+    w = np.array([1, 4, 1, 0]) if elfs else np.array([1, 2, 1, 0])
+    nnew = int(np.sum(w[act])) if n else 0
     ins = [np.ascontiguousarray(fields[k], np.float64) for k in names]
     outs = [np.zeros(max(nnew, 1)) for _ in names]
     dpp = C.POINTER(C.c_double) * len(names)
@@ -171,12 +173,12 @@ def ms_execute(prm: dict, act, partner, fields: dict, label, level):
     P = params_struct(prm)
     ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
     L = lib()
-    L.or_ms_execute.restype = C.c_int
-    L.or_ms_execute.argtypes = [C.POINTER(OrParams), C.c_int, ip, ip, dpp, ip, ip,
-                                dpp, ip, ip]
-    got = L.or_ms_execute(C.byref(P), n, act, np.ascontiguousarray(partner, np.int32),
-                          pin, np.ascontiguousarray(label, np.int32),
-                          np.ascontiguousarray(level, np.int32), pout, nlabel, nlevel)
+    fn = L.or_elfs_execute if elfs else L.or_ms_execute
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(OrParams), C.c_int, ip, ip, dpp, ip, ip, dpp, ip, ip]
+    got = fn(C.byref(P), n, act, np.ascontiguousarray(partner, np.int32),
+             pin, np.ascontiguousarray(label, np.int32),
+             np.ascontiguousarray(level, np.int32), pout, nlabel, nlevel)
     assert got == nnew
     return ({k: o[:nnew] for k, o in zip(names, outs)}, nlabel[:nnew], nlevel[:nnew])
 
@@ -193,6 +195,25 @@ def ms_decide(prm: dict, dt, x, y, Ap, hp, vpx_dx, vpx_dy, vpy_dx, vpy_dy,
     lib().or_ms_decide(C.byref(P), n, dt, *arrs,
                        np.ascontiguousarray(level, np.int32), act, partner,
                        counts)
+    return act[:n], partner[:n], counts
+
+
+def elfs_decide(prm: dict, x, y, Ap, hp, vpx_dx, vpy_dy, vpx, vpy, level):
+    """decide_actions of the legacy ELFS rule (merge_split_ops.h:349-510) on
+    a snapshot: (action, partner, counts)."""
+    n = len(x)
+    act = np.zeros(max(n, 1), np.int32)
+    partner = np.zeros(max(n, 1), np.int32)
+    counts = np.zeros(2, np.int32)
+    P = params_struct(prm)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in
+            (x, y, Ap, hp, vpx_dx, vpy_dy, vpx, vpy)]
+    L = lib()
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    L.or_elfs_decide.argtypes = [C.POINTER(OrParams), C.c_int] + [_dp] * 8 + [ip] * 4
+    L.or_elfs_decide.restype = None
+    L.or_elfs_decide(C.byref(P), n, *arrs, np.ascontiguousarray(level, np.int32),
+                     act, partner, counts)
     return act[:n], partner[:n], counts
 
 

@@ -1,10 +1,12 @@
 """The 4M-particle configuration at its full size (2048 x 2048 cells,
 hx = 0.5, fractal slope, polygonal release; amsc_amd.scenarios.large_fractal):
-the first step bit for bit against the oracle, then properties that do not
-depend on size — P2G mass and momentum transfer to round-off every step
-(data/conservation_test/scripts/conservation_check.py:36, 1e-10), total
-particle mass unchanged through the merge/split steps (steps 10 and 20), and
-the lake at rest over the same topography staying at rest
+bit for bit against the oracle after the first step and after the first
+merge/split step that splits or merges particles (step 30; steps 10 and 20
+find nothing to do), with the same merge/split counts; P2G mass and
+momentum transfer to round-off every step
+(data/conservation_test/scripts/conservation_check.py:36, 1e-10) and total
+particle mass unchanged through the merge/split steps; and the lake at rest
+over the same topography staying at rest
 (data/well_balance/scripts/wb_error.py:72, 1e-10)."""
 import numpy as np
 import pytest
@@ -23,34 +25,39 @@ def bits(a):
 def test_large_fractal_4m():
     from amsc_amd.simulation import Simulation
     print("[full size] large fractal 4M", flush=True)
-    d = scenarios.large_fractal(nsteps=30)
+    d = scenarios.large_fractal(nsteps=40)
     prm = params_from_data(d)
     part = particles_from_data(d)
     assert len(part["x"]) >= 3_900_000
     g = Simulation(prm, part, d["Z"], capacity=3 * len(part["x"]))
-    g.set_aux(True)
     o = oracle.OracleSim(prm, part, d["Z"])
-    o.step()
-    g.step(1)
-    for k in ("x", "y", "vpx", "vpy", "hp", "F_11", "F_12", "F_22", "Ap", "Fb_x", "Fb_y"):
-        assert np.array_equal(bits(g.get(k)), bits(o.get(k))), k
-    for k in ("Mv", "vvxL", "vvyL"):
-        assert np.array_equal(bits(g.get(k)), bits(o.get(k))), k
-    del o
-    g.set_aux(False)
     m0 = float(np.sum(d["Mp"]))
     worst = 0.0
-    ms_steps = 0
-    for it in range(2, 23):
+    ops = 0
+    for it in range(1, 32):
+        aux = it in (1, 31)
+        g.set_aux(aux)
+        o.step()
         assert g.step(1) == 1
         em, eo, _ = g.conservation()
         worst = max(worst, em, eo)
-        ms_steps += int(prm["ms_on"] and it % prm["ms_interval"] == 0)
+        if it % prm["ms_interval"] == 0:
+            assert np.array_equal(g.ms_counts(), o.ms_counts()), it
+            ops += int(sum(o.ms_counts()))
+            print(f"[full size] step {it}: merge/split {o.ms_counts()}", flush=True)
+        if aux:
+            print(f"[full size] step {it}: bitwise check", flush=True)
+            assert g.num_particles == o.np
+            for k in ("x", "y", "vpx", "vpy", "hp", "F_11", "F_12", "F_22", "Ap",
+                      "Fb_x", "Fb_y", "Mp", "Vp"):
+                assert np.array_equal(bits(g.get(k)), bits(o.get(k))), (it, k)
+            for k in ("Mv", "vvxL", "vvyL"):
+                assert np.array_equal(bits(g.get(k)), bits(o.get(k))), (it, k)
+            assert np.array_equal(g.get_int("level"), o.get_int("level"))
+    assert ops > 0  # step 30 split particles
     assert worst < 1e-10
-    assert ms_steps >= 1
     Mp = g.get("Mp")
     assert abs(Mp.sum() - m0) <= 1e-12 * m0
-    assert np.isfinite(g.get("x")).all() and np.isfinite(g.get("hp")).all()
     g.close()
 
 

@@ -286,26 +286,40 @@ def test_crowded_tiles_bitwise():
 
 
 def test_large_landslide_properties():
-    """The ~2M-particle slope landslide: one step bit-for-bit against the
-    oracle, then 40 adaptive steps (merge/split every 10) with mass transfer
-    to round-off every step and total particle mass unchanged by merges."""
+    """The ~2M-particle slope landslide: bit for bit against the oracle after
+    the first step and after the first merge/split step that splits
+    particles (step 20), with the same counts; then 20 more adaptive steps
+    with mass transfer to round-off every step and total particle mass
+    unchanged by merge/split."""
     from amsc_amd.simulation import Simulation
     print("[sim] large landslide", flush=True)
     d = scenarios.slope_landslide(max_steps=41)
     prm = params_from_data(d)
     part = particles_from_data(d)
     g = Simulation(prm, part, d["Z"], capacity=3 * len(d["x"]))
-    g.set_aux(True)
     o = oracle.OracleSim(prm, part, d["Z"])
-    o.step()
-    g.step(1)
-    for k in ("x", "y", "vpx", "vpy", "hp", "F_11", "F_12", "F_22", "Ap", "Fb_x"):
-        assert np.array_equal(bits(g.get(k)), bits(o.get(k))), k
-    del o
-    g.set_aux(False)
     m0 = d["Mp"].sum()
     worst = 0.0
-    for _ in range(40):
+    ops = 0
+    for it in range(1, 22):
+        aux = it in (1, 21)
+        g.set_aux(aux)
+        o.step()
+        assert g.step(1) == 1
+        em, eo, _ = g.conservation()
+        worst = max(worst, em, eo)
+        if it % prm["ms_interval"] == 0:
+            assert np.array_equal(g.ms_counts(), o.ms_counts()), it
+            ops += int(sum(o.ms_counts()))
+        if aux:
+            assert g.num_particles == o.np
+            for k in ("x", "y", "vpx", "vpy", "hp", "F_11", "F_12", "F_22", "Ap",
+                      "Fb_x", "Mp", "Vp"):
+                assert np.array_equal(bits(g.get(k)), bits(o.get(k))), (it, k)
+    assert ops > 0  # step 20 split particles
+    del o
+    g.set_aux(False)
+    for _ in range(19):
         assert g.step(1) == 1
         em, eo, _ = g.conservation()
         worst = max(worst, em, eo)
