@@ -132,6 +132,7 @@ class BatchedServer:
         repacks = 0
         repack_s = 0.0
         last_log = 0.0
+        peak = 0  # largest device footprint admitted (old + new batch)
         while pending or active:
             now = clock() - t0
             if now > budget_s:
@@ -149,6 +150,7 @@ class BatchedServer:
                 r = pending.pop(0)
                 r.admitted = now
                 joining.append(r)
+                peak = max(peak, need)
             if joining:
                 t_r = clock()
                 if ens is not None:
@@ -249,7 +251,8 @@ class BatchedServer:
         return {"form": f"batched-{self.precision}", "requests": len(requests),
                 "finished": fin, "diverged": int(sum(r.diverged for r in requests)),
                 "wall_s": wall, "scen_per_s": fin / wall,
-                "repacks": repacks, "repack_s": repack_s, **_pcts(tts)}
+                "repacks": repacks, "repack_s": repack_s, "peak_bytes": int(peak),
+                "budget_bytes": self.budget, **_pcts(tts)}
 
 
 class SequentialServer:

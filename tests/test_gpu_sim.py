@@ -97,6 +97,25 @@ def test_reference_fixture_bitwise():
     assert_same(o, g, "step 100")
 
 
+@pytest.mark.parametrize("n", [3000, 10000, 30000])
+def test_scaling_fixtures_bitwise(n):
+    """The reference's other scaling inputs, data/performance/
+    DATA_bench_{3000,10000,30000}.json (27^2, 50^2, 87^2 cells; 2500, 9216,
+    28900 particles): 100 fixed-dt steps with merge/split on, bit for bit,
+    the merge/split counts equal at every merge/split step."""
+    print(f"[sim] DATA_bench_{n}", flush=True)
+    d = load_data_json(os.path.join(GOLDEN, f"DATA_bench_{n}.json"))
+    o, g, prm = pair(d, capacity=4 * len(d["x"]))
+    g.set_aux(False)
+    for it in range(1, 101):
+        if it == 100:
+            g.set_aux(True)
+        advance(o, g, 1)
+        if it % prm["ms_interval"] == 0:
+            assert np.array_equal(g.ms_counts(), o.ms_counts()), it
+    assert_same(o, g, f"DATA_bench_{n} step 100")
+
+
 @pytest.mark.parametrize("case,steps", [("conservation", 80), ("well_balance", 60),
                                         ("fena", 40), ("dambreak", 60)])
 def test_reference_cases_bitwise(case, steps):
