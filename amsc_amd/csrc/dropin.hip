@@ -52,8 +52,15 @@ int dev_cache_get(DevCache &c, size_t bytes, void **out, const char *what) {
 }
 
 // Colour of each cell, rebuilt from (ccidx, coloff) on every scatter call.
+// One map serves every call; a call holds g_colour_mu from the rebuild to
+// the launch of the gather that reads it, and the rebuild waits (on the
+// caller's stream) for the last gather that read the map, on whatever
+// stream that gather was enqueued: calls that only enqueue their kernels,
+// on different streams, never overwrite a map that is still being read.
 DevCache g_colour;
 std::mutex g_colour_mu;
+hipEvent_t g_colour_read = nullptr;  // the last gather that read the map
+int g_colour_read_dev = -1;
 
 __global__ void fill_u8_kernel(uint8_t *p, int n, uint8_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -70,6 +77,7 @@ __global__ void colour_map_kernel(const int *ccidx, int nc, int o1, int o2,
 }
 
 // Validate the colouring and build the colour map; returns ncols in *ncols.
+// The caller holds g_colour_mu until the map's reader is enqueued.
 int prepare_colouring(int nrows, const int *ccidx, const int *coloff,
                       int *ncols, const uint8_t **colour) {
   if (nrows <= 0) return fail(AMSC_ERR_INVALID, "nrows must be positive");
@@ -88,12 +96,13 @@ int prepare_colouring(int nrows, const int *ccidx, const int *coloff,
     *colour = nullptr;
     return AMSC_OK;
   }
-  std::lock_guard<std::mutex> lk(g_colour_mu);
   void *p = nullptr;
   if (int rc = dev_cache_get(g_colour, (size_t)nc, &p, "colour map"))
     return rc;
   uint8_t *map = static_cast<uint8_t *>(p);
   hipStream_t st = current_stream();
+  if (g_colour_read && g_colour_read_dev == g_colour.dev)
+    AMSC_HIP(hipStreamWaitEvent(st, g_colour_read, 0));
   fill_u8_kernel<<<grid_for(nc, kBlock), kBlock, 0, st>>>(map, nc, 0xFF);
   colour_map_kernel<<<grid_for(nc, kBlock), kBlock, 0, st>>>(
       ccidx, nc, coloff[1], coloff[2], coloff[3], map);
@@ -269,14 +278,24 @@ int tiled_gather(int nrows, double hx, double hy, const int *cstart,
                  const double *x, const double *y, TGFields<NIN, NOUT> f) {
   int ncols = 0;
   const uint8_t *colour = nullptr;
+  std::lock_guard<std::mutex> lk(g_colour_mu);
   if (int rc = prepare_colouring(nrows, ccidx, coloff, &ncols, &colour))
     return rc;
   if (ncols == 0) return AMSC_OK;
   const int ntx = (ncols + 1 + kTC - 1) / kTC, nty = (nrows + 1 + kTR - 1) / kTR;
+  hipStream_t st = current_stream();
   tiled_gather_kernel<NIN, NOUT, INTERNAL>
-      <<<ntx * nty, kTNT, 0, current_stream()>>>(nrows, ncols, hx, hy, ntx,
-                                                 cstart, cptcls, colour, x, y, f);
+      <<<ntx * nty, kTNT, 0, st>>>(nrows, ncols, hx, hy, ntx, cstart, cptcls,
+                                   colour, x, y, f);
   AMSC_LAUNCH_CHECK();
+  // the map's last reader, for the next rebuild to wait for
+  if (g_colour_read_dev != g_colour.dev) {
+    if (g_colour_read) (void)hipEventDestroy(g_colour_read);
+    g_colour_read = nullptr;
+    AMSC_HIP(hipEventCreateWithFlags(&g_colour_read, hipEventDisableTiming));
+    g_colour_read_dev = g_colour.dev;
+  }
+  AMSC_HIP(hipEventRecord(g_colour_read, st));
   return AMSC_OK;
 }
 
