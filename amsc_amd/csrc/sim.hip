@@ -62,6 +62,10 @@ constexpr int kP2GVariants = 2 * kTileCfgs;
 constexpr int kTileTC[kTileCfgs] = {16, 16, 8};
 constexpr int kTileTR[kTileCfgs] = {32, 16, 32};
 constexpr int kPartBS[2] = {256, 512};
+// A scenario whose cells have held more than this many particles runs its
+// P2G on the dense-cell kernels (at 4 particles per cell the tiled P2G is
+// the faster; from a few dozen per cell it serialises on the crowded cells).
+constexpr int kDensePPC = 24;
 constexpr int kMaxScen = 1 << 16;
 
 struct TimedLaunch {
@@ -119,6 +123,14 @@ struct Sim {
   // default tiling: the fastest on every workload measured (8x32-node P2G
   // tiles in linear order, 512-thread particle workgroups)
   int tile_cfg = 2, tile_order = 1, part_bs = 512;
+  // scenarios whose P2G runs on the dense-cell kernels (decided on the host
+  // from the control blocks' max_ppc, once a cell has held more than
+  // kDensePPC particles), their flags on the device, and the staged values
+  // those kernels use (allocated when the first scenario turns dense)
+  std::vector<char> dense_h;
+  int *dense_d = nullptr;
+  bool any_dense = false;
+  P2GStaged staged{};
   long long particle_steps = 0;
   size_t bytes = 0;  // device memory requested (cached blocks may be larger)
   std::vector<std::pair<void *, size_t>> allocs;  // every dalloc block
@@ -539,7 +551,29 @@ int launch_p2g_order(Sim &s, const P2GArgs &a) {
 }
 
 template <typename R>
-int launch_p2g(Sim &s, const P2GArgs &a) {
+int launch_p2g_dense(Sim &s, const P2GArgs &a, int npos) {
+  if (npos > 0)
+    if (int rc = launch(s, "p2g_stage", p2g_stage_kernel<R>,
+                        dim3(grid_for(npos, 256)), dim3(256), a))
+      return rc;
+  const dim3 g(s.ntiles[s.tile_cfg]);
+  switch (s.tile_cfg) {
+  case 0:
+    return launch(s, "p2g_dense", p2g_dense_node_kernel<R, 16, 32>, g, dim3(512), a);
+  case 1:
+    return launch(s, "p2g_dense", p2g_dense_node_kernel<R, 16, 16>, g, dim3(256), a);
+  default:
+    return launch(s, "p2g_dense", p2g_dense_node_kernel<R, 8, 32>, g, dim3(256), a);
+  }
+}
+
+// P2G of every active scenario: the dense-cell kernels for the scenarios
+// flagged dense (npos bounds the binned positions), the tiled kernel for
+// the others.
+template <typename R>
+int launch_p2g(Sim &s, const P2GArgs &a, int npos) {
+  if (a.dense)
+    if (int rc = launch_p2g_dense<R>(s, a, npos)) return rc;
   return s.tile_order == 0 ? launch_p2g_order<R, true>(s, a)
                            : launch_p2g_order<R, false>(s, a);
 }
@@ -571,6 +605,9 @@ P2GArgs p2g_args(Sim &s, int aux, bool partials) {
   a.vvyL = s.nod.f[NF_VVYL];
   a.partials = partials ? s.partials : nullptr;
   a.out = s.p2g_out[s.cur];
+  a.dense = s.any_dense ? s.dense_d : nullptr;
+  a.key = s.key[s.kcur];
+  a.st = s.staged;
   return a;
 }
 
@@ -615,9 +652,9 @@ int step_body(Sim &s, bool ms_step, int aux, bool capture, bool *skipped) {
   }
   const Tables T = tables(s);
   PPtr &p = s.buf[s.cur];
-  if (int rc = launch_p2g<R>(s, p2g_args(s, aux, s.diag))) return rc;
-
   const int n = capture ? s.cap : s.np;
+  if (int rc = launch_p2g<R>(s, p2g_args(s, aux, s.diag), n)) return rc;
+
   if (n > 0) {
     PartArgs a;
     a.T = T;
@@ -653,7 +690,7 @@ int launch_graph(Sim &s, bool ms_step, int aux) {
                        (long long)s.tile_order << 7 |
                        (long long)(s.part_bs == 512) << 12 |
                        (long long)s.diag << 9 | (long long)s.mom_stored << 10 |
-                       (long long)s.aux_on << 11;
+                       (long long)s.aux_on << 11 | (long long)s.any_dense << 13;
   auto it = s.graphs.find(gk);
   if (it == s.graphs.end()) {
     const int c0 = s.cur, k0 = s.kcur;
@@ -747,9 +784,35 @@ int launch_step(Sim &s) {
   return AMSC_OK;
 }
 
+// Flag for the dense-cell P2G the scenarios whose cells have held more than
+// kDensePPC particles (from the control blocks just read); a flag, once
+// set, stays.  The staged-value buffer is allocated with the first flag.
+int update_dense(Sim &s) {
+  bool changed = false;
+  for (int i = 0; i < s.S; ++i)
+    if (!s.dense_h[i] && s.ctrl_host[i].max_ppc > kDensePPC) {
+      s.dense_h[i] = 1;
+      changed = true;
+    }
+  if (!changed) return AMSC_OK;
+  if (!s.staged.rec) {
+    const size_t rb = s.mixed ? sizeof(P2GRec<float>) : sizeof(P2GRec<double>);
+    char *q = nullptr;
+    if (int rc = dalloc(s, &q, rb * (size_t)s.cap)) return rc;
+    s.staged.rec = q;
+  }
+  std::vector<int> h(s.dense_h.begin(), s.dense_h.end());
+  AMSC_HIP(hipMemcpyAsync(s.dense_d, h.data(), sizeof(int) * s.S,
+                          hipMemcpyHostToDevice, s.st));
+  AMSC_HIP(hipStreamSynchronize(s.st));
+  s.any_dense = true;
+  return AMSC_OK;
+}
+
 int step_n(Sim &s, int nsteps, int *done) {
   if (nsteps < 0) return fail(AMSC_ERR_INVALID, "negative step count");
   if (int rc = read_ctrl(s)) return rc;
+  if (int rc = update_dense(s)) return rc;
   std::vector<int> it0 = s.it_host;
   for (int k = 0; k < nsteps && !s.finished; ++k)
     if (int rc = launch_step(s)) return rc;
@@ -902,6 +965,9 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   }
   if (int rc = dalloc(*s, &s->Z, s->nn)) return rc;
   if (int rc = dalloc(*s, &s->ctrl, S)) return rc;
+  if (int rc = dalloc(*s, &s->dense_d, S)) return rc;
+  s->dense_h.assign(S, 0);
+  if (int rc = launch_fill_int(*s, s->dense_d, S, 0)) return rc;
   int maxt = 0;
   for (int c = 0; c < kTileCfgs; ++c) maxt = std::max(maxt, s->ntiles[c]);
   if (int rc = dalloc(*s, &s->partials, 6 * (size_t)maxt)) return rc;
@@ -1457,7 +1523,7 @@ int time_p2g(Sim &s, int reps, double *avg_ms) {
   int rc = AMSC_OK;
   if (hipEventRecord(e0, s.st) != hipSuccess) rc = fail(AMSC_ERR_HIP, "event");
   for (int i = 0; i < reps && !rc; ++i)
-    rc = s.mixed ? launch_p2g<float>(s, a) : launch_p2g<double>(s, a);
+    rc = s.mixed ? launch_p2g<float>(s, a, s.np) : launch_p2g<double>(s, a, s.np);
   if (!rc && hipEventRecord(e1, s.st) != hipSuccess) rc = fail(AMSC_ERR_HIP, "event");
   float ms = 0.f;
   if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
@@ -1484,7 +1550,11 @@ long long bytes_estimate(int S, const amsc_params *prm, const int *np,
   }
   const long long cap = capacity > 0 ? std::max(capacity, npt) : 2 * std::max(npt, 1ll);
   const long long r = precision == AMSC_PREC_MIXED ? 4 : 8;
-  const long long per_p = 2 * (4 * 8 + (PF_COUNT - 4) * r + 8) + 7 * 4 + 4;
+  // two particle buffers, the int arrays, and the dense-cell P2G's staged
+  // values (allocated only if a scenario turns dense: counted anyway)
+  const long long per_p = 2 * (4 * 8 + (PF_COUNT - 4) * r + 8) + 7 * 4 + 4 +
+                          (r == 8 ? (long long)sizeof(P2GRec<double>)
+                                  : (long long)sizeof(P2GRec<float>));
   const long long per_n = 2 * r + (NF_COUNT - 2) * 8 + 8;
   bool elfs = false;
   for (int i = 0; i < S; ++i) elfs = elfs || prm[i].ms_mode == 1;

@@ -92,7 +92,10 @@ struct Ctrl {
   // fric_set: the host asks the next step to read the stored field;
   // fric_read: the prologue's copy of it for the step being taken.
   int fric_set, fric_read;
-  int pad;
+  // largest particle count of a cell in any binning since creation (the
+  // particle update and re-binning raise it); the host flags a scenario for
+  // the dense-cell P2G from it
+  int max_ppc;
   // max |vpx|, |vpy|, hp of the particles as bit patterns of non-negative
   // doubles, by step parity, spread over 64 addresses so that the per-block
   // atomics of the particle kernel do not all hit one line
@@ -338,6 +341,8 @@ __global__ void key_kernel(Tables T, const double *__restrict__ x,
   if (valid) {
     key[ip] = k;
     slot[ip] = sl;
+    Ctrl &C = T.ctrl[scen_of(T, ip)];
+    if (sl + 1 > C.max_ppc) atomicMax(&C.max_ppc, sl + 1);
   }
 }
 
@@ -349,10 +354,27 @@ struct P2GOut {
   double *nod[NF_COUNT];                       // nodal set (FP64 ones)
 };
 
+// The staged P2G values of one particle for the dense-cell P2G: FP64 local
+// coordinates and momentum, then Mp, Fpx, Fpy, frx, fry, F11, F12, F22, Vp
+// of type R; one record per binned position (of cptcls), 16-byte aligned so
+// that a node thread reads a record with a few 16-byte loads.
+template <typename R> struct alignas(16) P2GRec {
+  double xi, eta, mx, my;
+  R v[9];
+};
+struct P2GStaged {
+  void *rec;  // P2GRec<R>[capacity]
+};
+
 struct P2GArgs {
   Tables T;
   const int *tscen;
   int aux, mom_stored;
+  // per scenario: 1 = P2G by the dense-cell kernels (the tiled P2G skips
+  // its tiles); null = no scenario is
+  const int *dense;
+  const int *key;  // binned cell of each particle
+  P2GStaged st;
   const int *cstart, *cptcls;
   const double *x, *y, *hp, *H;
   // type R: Mp, momentum source (vpx/vpy, or the DATA momentum on the first
@@ -404,6 +426,57 @@ __device__ __forceinline__ void p2g_fields(const P2GArgs &a, int ip, int fip,
   W.f22 = static_cast<const R *>(a.f22)[ip];
 }
 
+// Nodal update of node (c, r) from its finished P2G sums: force balance and
+// nodal velocities (optim_par.cpp:467-497, D1), boundary conditions
+// (optim_par.cpp:503-529: nodes of the first/last cell column lose their x
+// components, of the first/last cell row their y components), the
+// velocities vL for G2P and, on output steps, the whole nodal set.
+template <typename R>
+__device__ __forceinline__ void p2g_node_update(
+    const P2GArgs &a, int n, int c, int r, int nrows, int ncols, int bc,
+    double dt, double Mv, double mvx, double mvy, Arith<R> FPx, Arith<R> FPy,
+    Arith<R> Frx, Arith<R> Fry, Arith<R> Fix, Arith<R> Fiy) {
+  using A = Arith<R>;
+  const A Fex = FPx + Frx, Fey = FPy + Fry;
+  const A Ftx = Fex - Fix, Fty = Fey - Fiy;
+  double mx = dt * (double)Ftx + mvx, my = dt * (double)Fty + mvy;
+  double ax = Mv > 1.e-2 ? (double)Ftx / Mv : 0.0;
+  double ay = Mv > 1.e-2 ? (double)Fty / Mv : 0.0;
+  double vx = Mv > 1.e-2 ? mx / Mv : 0.0;
+  double vy = Mv > 1.e-2 ? my / Mv : 0.0;
+  double vxL = dt * ax + vx, vyL = dt * ay + vy;
+  if (bc) {
+    if (c == 0 || c == 1 || c == ncols - 1 || c == ncols) {
+      ax = 0.0; vx = 0.0; vxL = 0.0; mx = 0.0;
+    }
+    if (r == 0 || r == 1 || r == nrows - 1 || r == nrows) {
+      ay = 0.0; vy = 0.0; vyL = 0.0; my = 0.0;
+    }
+  }
+  static_cast<R *>(a.vvxL)[n] = (R)vxL;
+  static_cast<R *>(a.vvyL)[n] = (R)vyL;
+  if (a.aux >= AUX_FULL) {
+    double *const *nod = a.out->nod;
+    global_ptr<double>(nod[NF_AVX])[n] = ax;
+    global_ptr<double>(nod[NF_AVY])[n] = ay;
+    global_ptr<double>(nod[NF_MV])[n] = Mv;
+    global_ptr<double>(nod[NF_MOMX])[n] = mx;
+    global_ptr<double>(nod[NF_MOMY])[n] = my;
+    global_ptr<double>(nod[NF_FEXTX])[n] = Fex;
+    global_ptr<double>(nod[NF_FEXTY])[n] = Fey;
+    global_ptr<double>(nod[NF_FINTX])[n] = Fix;
+    global_ptr<double>(nod[NF_FINTY])[n] = Fiy;
+    global_ptr<double>(nod[NF_FRX])[n] = Frx;
+    global_ptr<double>(nod[NF_FRY])[n] = Fry;
+    global_ptr<double>(nod[NF_FPX])[n] = FPx;
+    global_ptr<double>(nod[NF_FPY])[n] = FPy;
+    global_ptr<double>(nod[NF_FTOTX])[n] = Ftx;
+    global_ptr<double>(nod[NF_FTOTY])[n] = Fty;
+    global_ptr<double>(nod[NF_VVX])[n] = vx;
+    global_ptr<double>(nod[NF_VVY])[n] = vy;
+  }
+}
+
 // Staged particle i lives at i + i/16 in each LDS array: the node threads of
 // a wavefront walk neighbouring cells in lockstep, i.e. at a stride of about
 // 4 particles, which without the pad maps 16 lanes onto one bank.
@@ -441,7 +514,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   // prefetched fields
   const KParams Pv = sc.P;
   Ctrl *ctrl = &a.T.ctrl[sid];
-  if (!ctrl->active) return;
+  if (!ctrl->active || (a.dense && a.dense[sid])) return;
   // stored friction force (first step, after merge/split or a field
   // write) or Ap * Fb recomputed from the stored vpx, vpy, hp, Vp
   const bool fric_read = ctrl->fric_read || ms_due(sc, *ctrl);
@@ -659,51 +732,9 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     }
   }
 
-  if (node_ok && (L > 0 || a.aux >= AUX_FULL)) {
-    const int n = noff + r + c * (nrows + 1);
-    const double dt = ctrl->dt;
-    // force balance and nodal velocities, optim_par.cpp:467-497 (D1)
-    const A Fex = FPx + Frx, Fey = FPy + Fry;
-    const A Ftx = Fex - Fix, Fty = Fey - Fiy;
-    double mx = dt * (double)Ftx + mvx, my = dt * (double)Fty + mvy;
-    double ax = Mv > 1.e-2 ? (double)Ftx / Mv : 0.0;
-    double ay = Mv > 1.e-2 ? (double)Fty / Mv : 0.0;
-    double vx = Mv > 1.e-2 ? mx / Mv : 0.0;
-    double vy = Mv > 1.e-2 ? my / Mv : 0.0;
-    double vxL = dt * ax + vx, vyL = dt * ay + vy;
-    // boundary conditions, optim_par.cpp:503-529 (nodes of the first/last
-    // cell column: x components; first/last cell row: y components)
-    if (bc) {
-      if (c == 0 || c == 1 || c == ncols - 1 || c == ncols) {
-        ax = 0.0; vx = 0.0; vxL = 0.0; mx = 0.0;
-      }
-      if (r == 0 || r == 1 || r == nrows - 1 || r == nrows) {
-        ay = 0.0; vy = 0.0; vyL = 0.0; my = 0.0;
-      }
-    }
-    static_cast<R *>(a.vvxL)[n] = (R)vxL;
-    static_cast<R *>(a.vvyL)[n] = (R)vyL;
-    if (a.aux >= AUX_FULL) {
-      double *const *nod = a.out->nod;
-      global_ptr<double>(nod[NF_AVX])[n] = ax;
-      global_ptr<double>(nod[NF_AVY])[n] = ay;
-      global_ptr<double>(nod[NF_MV])[n] = Mv;
-      global_ptr<double>(nod[NF_MOMX])[n] = mx;
-      global_ptr<double>(nod[NF_MOMY])[n] = my;
-      global_ptr<double>(nod[NF_FEXTX])[n] = Fex;
-      global_ptr<double>(nod[NF_FEXTY])[n] = Fey;
-      global_ptr<double>(nod[NF_FINTX])[n] = Fix;
-      global_ptr<double>(nod[NF_FINTY])[n] = Fiy;
-      global_ptr<double>(nod[NF_FRX])[n] = Frx;
-      global_ptr<double>(nod[NF_FRY])[n] = Fry;
-      global_ptr<double>(nod[NF_FPX])[n] = FPx;
-      global_ptr<double>(nod[NF_FPY])[n] = FPy;
-      global_ptr<double>(nod[NF_FTOTX])[n] = Ftx;
-      global_ptr<double>(nod[NF_FTOTY])[n] = Fty;
-      global_ptr<double>(nod[NF_VVX])[n] = vx;
-      global_ptr<double>(nod[NF_VVY])[n] = vy;
-    }
-  }
+  if (node_ok && (L > 0 || a.aux >= AUX_FULL))
+    p2g_node_update<R>(a, noff + r + c * (nrows + 1), c, r, nrows, ncols, bc,
+                       ctrl->dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry, Fix, Fiy);
 
   if (a.partials) {
     // conservation sums (optim_par.cpp:391-428), FP64, fixed order: tile
@@ -711,6 +742,193 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     const int lane = t & 63, w = t >> 6;
     const double v[6] = {pm, ppx, ppy, node_ok ? Mv : 0.0, node_ok ? mvx : 0.0,
                          node_ok ? mvy : 0.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double sm = wave_sum(v[k]);
+      if (lane == 0) s_red[k][w] = sm;
+    }
+    __syncthreads();
+    if (t < 6) {
+      double sm = 0.0;
+      for (int ww = 0; ww < NT / 64; ++ww) sm += s_red[t][ww];
+      a.partials[tile * 6 + t] = sm;
+    }
+  }
+}
+
+// ------------------------------------------------------------- dense P2G
+// P2G for scenarios whose cells hold many particles (a deposit piling up at
+// the foot of a slope).  The tiled P2G stages its tile's particles through
+// LDS a chunk at a time and only the nodes beside a chunk's few cells have
+// work, so a tile of crowded cells runs for many chunks with most threads
+// idle.  Here the staging is done once per particle into a buffer in
+// binned order (p2g_stage_kernel), and then one thread per node walks its
+// four cells in that buffer (p2g_dense_node_kernel): all nodes fold at
+// once.  Same staging and folding expressions, same order per node (cells
+// ascending, particles ascending), so the sums are the tiled P2G's bit for
+// bit.
+
+// One thread per binned position q: the staged values of particle
+// cptcls[q] of a dense scenario, as the tiled P2G stages them, and on
+// output steps the particle's output fields.
+template <typename R>
+__global__ void __launch_bounds__(256) p2g_stage_kernel(P2GArgs a) {
+  using A = Arith<R>;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.T.poff[a.T.S]) return;
+  const int ip = a.cptcls[q];
+  const int sid = scen_of(a.T, ip);
+  if (!a.dense[sid]) return;
+  const Scen &sc = a.T.scen[sid];
+  const Ctrl &C = a.T.ctrl[sid];
+  if (!C.active) return;
+  const bool fric_read = C.fric_read || ms_due(sc, C);
+  const KParams &P = sc.P;
+  const int nrows = P.nrows;
+  const double hx = P.hx, hy = P.hy, grav = P.g, rho = P.rho;
+  const Recip rc = sc.rc;
+  const int cell = a.key[ip] - sc.coff, c = fdiv(cell, sc.fd), r = cell - c * nrows;
+  P2GRaw<R> W;
+  p2g_fields<R>(a, ip, fric_read ? ip : 0, W);
+  double z[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) z[k] = a.Z[sc.noff + node_id(k, c, r, nrows)];
+  const Local Lc = local_coords(W.x, W.y, c, r, hx, hy, rc);
+  // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
+  double dzx = 0.0, dzy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    dzx += shape_dx(Lc, k, rc) * z[k];
+    dzy += shape_dy(Lc, k, rc) * z[k];
+  }
+  // gravity with the well-balance factor, optim_par.cpp:450-453
+  const double wb = W.hp > 1e-10 ? 1.0 - (W.H / W.hp) : 1.0;
+  const double Vpd = W.Vp;
+  const double Fpx = -grav * Vpd * rho * dzx * wb;
+  const double Fpy = -grav * Vpd * rho * dzy * wb;
+  // mom_p = vpx * Mp (optim_par.cpp:600-601), DATA momentum at step 1
+  const double Mpd = W.Mp;
+  const double mx = a.mom_stored ? (double)W.m1 : (double)W.m1 * Mpd;
+  const double my = a.mom_stored ? (double)W.m2 : (double)W.m2 * Mpd;
+  R frx = W.frx, fry = W.fry;
+  if (!fric_read) {
+    A fbx, fby, ap;
+    voellmy<A>(P, (A)W.m1, (A)W.m2, W.hp, (A)W.Vp, fbx, fby, ap);
+    frx = ap * fbx;
+    fry = ap * fby;
+  }
+  P2GRec<R> rec;
+  rec.xi = Lc.xi;
+  rec.eta = Lc.eta;
+  rec.mx = mx;
+  rec.my = my;
+  const R v[9] = {W.Mp, (R)Fpx, (R)Fpy, frx, fry, W.f11, W.f12, W.f22, W.Vp};
+#pragma unroll
+  for (int k = 0; k < 9; ++k) rec.v[k] = v[k];
+  static_cast<P2GRec<R> *>(a.st.rec)[q] = rec;
+  if (a.aux >= AUX_FULL) {
+    const P2GOut *o = a.out;
+    global_ptr<R>(o->dzx)[ip] = (R)dzx;
+    global_ptr<R>(o->dzy)[ip] = (R)dzy;
+    global_ptr<R>(o->fpx)[ip] = (R)Fpx;
+    global_ptr<R>(o->fpy)[ip] = (R)Fpy;
+    global_ptr<R>(o->momx)[ip] = (R)mx;
+    global_ptr<R>(o->momy)[ip] = (R)my;
+  }
+}
+
+// One thread per node of a dense scenario's tiles (the tile layout of the
+// tiled P2G, whose conservation partials it writes in the same places):
+// the node's four cells in ascending cell id, each cell's staged particles
+// in ascending id, then the nodal update.
+template <typename R, int TC, int TR>
+__global__ void __launch_bounds__(TC * TR)
+    __attribute__((amdgpu_waves_per_eu(1, 2))) p2g_dense_node_kernel(P2GArgs a) {
+  constexpr int NT = TC * TR, D = 8;
+  using A = Arith<R>;
+  const int tile = blockIdx.x;
+  const int sid = __builtin_amdgcn_readfirstlane(a.tscen[tile]);
+  if (!a.dense[sid]) return;
+  const Scen &sc = a.T.scen[sid];
+  const Ctrl *ctrl = &a.T.ctrl[sid];
+  if (!ctrl->active) return;
+  const int nrows = sc.P.nrows, ncols = sc.P.ncols;
+  const RecipT<A> rcr = recip_of<A>(sc);
+  const int lt = tile - sc.toff;
+  const int c0 = (lt % sc.ntx) * TC, r0 = (lt / sc.ntx) * TR;
+  const int t = threadIdx.x;
+  const int c = c0 + t / TR, r = r0 + t % TR;
+  const bool node_ok = c <= ncols && r <= nrows;
+  const int *cstart = a.cstart + sc.coff;
+  const P2GRec<R> *recs = static_cast<const P2GRec<R> *>(a.st.rec);
+  double pm = 0, ppx = 0, ppy = 0;  // particle sums (the node's own cell)
+  double Mv = 0, mvx = 0, mvy = 0;
+  A FPx = 0, FPy = 0, Frx = 0, Fry = 0, Fix = 0, Fiy = 0;
+  if (node_ok) {
+    // cells (c-1,r-1) corner 3, (c-1,r) corner 2, (c,r-1) corner 1, (c,r)
+    // corner 0: ascending cell id
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cc = c - 1 + (k >> 1), cr = r - 1 + (k & 1);
+      if (cc < 0 || cc >= ncols || cr < 0 || cr >= nrows) continue;
+      const int corner = 3 - k;
+      const int b = cstart[cc * nrows + cr], e = cstart[cc * nrows + cr + 1];
+      // D particles' values are loaded together (every load issued before
+      // the first is used: a chain of one memory round trip per particle
+      // would be latency-bound), then folded in order
+      for (int q0 = b; q0 < e; q0 += D) {
+        const int n = e - q0 < D ? e - q0 : D;
+        P2GRec<R> rr[D];
+#pragma unroll
+        for (int u = 0; u < D; ++u) rr[u] = recs[q0 + (u < n ? u : 0)];
+        // an (empty) use of every loaded value here, so that the compiler
+        // keeps the D records' loads together instead of sinking each into
+        // the iteration that needs it
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+          asm volatile("" ::"v"(rr[u].xi), "v"(rr[u].eta), "v"(rr[u].mx), "v"(rr[u].my));
+#pragma unroll
+          for (int f = 0; f < 9; ++f) asm volatile("" ::"v"(rr[u].v[f]));
+        }
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+          if (u >= n) break;
+          const R *fv = rr[u].v;
+          const double xi = rr[u].xi, eta = rr[u].eta;
+          const double Mp = (double)fv[0], mx = rr[u].mx, my = rr[u].my;
+          const double N = shape_t<double>(xi, eta, corner);
+          Mv += N * Mp;
+          mvx += N * mx;
+          mvy += N * my;
+          const A Nr = (A)N;
+          FPx += Nr * (A)fv[1];
+          FPy += Nr * (A)fv[2];
+          Frx += Nr * (A)fv[3];
+          Fry += Nr * (A)fv[4];
+          const A xr = (A)xi, er = (A)eta;
+          const A Nx = shape_dx_t<A>(xr, er, corner, rcr);
+          const A Ny = shape_dy_t<A>(xr, er, corner, rcr);
+          const A f11 = (A)fv[5], f12 = (A)fv[6], f22 = (A)fv[7];
+          const A vp = fv[8];
+          Fix += (Nx * f11 + Ny * f12) * vp;
+          Fiy += (Nx * f12 + Ny * f22) * vp;
+          if (k == 3) {
+            pm += Mp;
+            ppx += mx;
+            ppy += my;
+          }
+        }
+      }
+    }
+    p2g_node_update<R>(a, sc.noff + r + c * (nrows + 1), c, r, nrows, ncols,
+                       sc.P.bc, ctrl->dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry,
+                       Fix, Fiy);
+  }
+  if (a.partials) {
+    // conservation sums, FP64: tile partials as the tiled P2G writes them
+    __shared__ double s_red[6][NT / 64];
+    const int lane = t & 63, w = t >> 6;
+    const double v[6] = {pm, ppx, ppy, Mv, mvx, mvy};
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const double sm = wave_sum(v[k]);
@@ -992,6 +1210,20 @@ __global__ void __launch_bounds__(BS)
   if (valid) {
     a.key_out[ip] = newkey;
     a.slot[ip] = sl;
+  }
+  // the largest cell count of this binning (a slot + 1 is a count its cell
+  // reaches), raised in the control block only past the value stored there
+  if (EDGE) {
+    if (valid && sl + 1 > T.ctrl[sid].max_ppc) atomicMax(&T.ctrl[sid].max_ppc, sl + 1);
+  } else {
+    int m = valid ? sl + 1 : 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int o = __shfl_xor(m, d);
+      m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m > T.ctrl[sid].max_ppc)
+      atomicMax(&T.ctrl[sid].max_ppc, m);
   }
   if (EDGE) {
     if (valid && act) {

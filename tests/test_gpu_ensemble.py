@@ -422,3 +422,39 @@ def test_empty_scenario_in_a_batch():
     s.step(5)
     assert s.num_particles == 0 and s.state()[0] == 5
     s.close()
+
+
+def test_dense_scenario_in_a_batch():
+    """A crowded scenario (about 200 particles per cell, so it runs the
+    dense-cell P2G after its first step) beside an ordinary one (tiled P2G)
+    in one batch, FP64 and mixed: each ends in the state of its own run
+    alone, bit for bit, and the FP64 batch in the oracle's."""
+    from amsc_amd.simulation import Ensemble
+    from test_gpu_sim import crowded_case
+    print("[ens] dense scenario in a batch", flush=True)
+    cases = [scen(crowded_case(n=12000, seed=3)), scen(small_slope_case())]
+    for precision in ("fp64", "mixed"):
+        cap = 4 * sum(len(p["x"]) for _, p, _ in cases)
+        e = Ensemble(cases, precision, capacity=cap)
+        e.step(1)
+        e.set_timing(True)
+        e.step(7)
+        kt = e.kernel_times()
+        e.set_timing(False)
+        assert "p2g_dense" in kt, kt
+        e.set_aux(True)
+        e.step(1)
+        for k, c in enumerate(cases):
+            a = Ensemble([c], precision, capacity=4 * len(c[1]["x"]))
+            a.step(8)
+            a.set_aux(True)
+            a.step(1)
+            for f in FIELDS:
+                assert np.array_equal(bits(a.get(0, f)), bits(e.get(k, f))), (precision, k, f)
+            a.close()
+            if precision == "fp64":
+                o = oracle.OracleSim(*c)
+                for _ in range(9):
+                    o.step()
+                assert_scen_equal(e, k, o, f"scenario {k}")
+        e.close()

@@ -285,23 +285,40 @@ def test_first_step_uses_data_momentum():
     assert_same(o, g, "step 3")
 
 
-def test_crowded_tiles_bitwise():
-    """~200 particles per cell in a 10x10-cell patch: P2G tiles stage their
-    particles through LDS in many chunks."""
-    print("[sim] crowded", flush=True)
-    rng = np.random.default_rng(8)
+def crowded_case(n=20000, seed=8):
+    rng = np.random.default_rng(seed)
     d = small_slope_case(nrows=40, ncols=40, hx=0.25, hy=0.25)
-    n = 20000
     x = rng.uniform(2.5, 5.0, n)
     y = rng.uniform(2.5, 5.0, n)
     hp = rng.uniform(0.2, 1.0, n)
     Ap = np.full(n, 0.25 * 0.25 / 200)
     d.update(x=x, y=y, hp=hp, Ap=Ap, Vp=hp * Ap, Mp=2000.0 * hp * Ap,
              vpx=np.zeros(n), vpy=np.zeros(n), mom_px=np.zeros(n),
-             mom_py=np.zeros(n), DT_FIXED=5e-4, NSTEPS=6)
-    o, g, _ = pair(d)
-    advance(o, g, 6)
+             mom_py=np.zeros(n), DT_FIXED=5e-4, NSTEPS=12)
+    return d
+
+
+def test_crowded_tiles_bitwise():
+    """~200 particles per cell in a 10x10-cell patch: the first step's P2G
+    stages each tile's particles through LDS in many chunks; from the
+    second step on (its cells have held more than 24 particles) the
+    scenario runs the dense-cell P2G (staged values in binned order, one
+    thread per node); both bit for bit."""
+    print("[sim] crowded", flush=True)
+    o, g, _ = pair(crowded_case())
+    advance(o, g, 1)
+    assert_same(o, g, "step 1")
+    g.set_timing(True)
+    advance(o, g, 5)
+    kt = g.kernel_times()
+    g.set_timing(False)
+    assert "p2g_dense" in kt and "p2g_stage" in kt, kt
     assert_same(o, g, "step 6")
+    g.set_aux(False)
+    advance(o, g, 5)
+    g.set_aux(True)
+    advance(o, g, 1)
+    assert_same(o, g, "step 12")
 
 
 def test_large_landslide_properties():
