@@ -85,13 +85,13 @@ __device__ __forceinline__ void cx(int &a, int &b) {
 }
 
 __global__ void cell_sort_kernel(int nc, const int *__restrict__ cstart,
-                                 int *cptcls, const int *__restrict__ active) {
+                                 int *cptcls, const int *__restrict__ active,
+                                 int *big, int big_cap) {
   if (active && !*active) return;
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= nc) return;
-  const int b = cstart[cell], e = cstart[cell + 1], n = e - b;
-  if (n < 2) return;
-  if (n <= 8) {
+  const int b = cell < nc ? cstart[cell] : 0;
+  const int n = cell < nc ? cstart[cell + 1] - b : 0;
+  if (n >= 2 && n <= 8) {
     // the common case (a few particles per cell): Batcher's odd-even merge
     // network for 8 keys (19 compare-exchanges), padded with INT_MAX
     int v[8];
@@ -110,8 +110,63 @@ __global__ void cell_sort_kernel(int nc, const int *__restrict__ cstart,
 #pragma unroll
     for (int i = 0; i < 8; ++i)
       if (i < n) cptcls[b + i] = v[i];
-  } else {
-    heap_sort(cptcls + b, n);
+    return;
+  }
+  if (n <= 8) return;
+  // a crowded cell: queued for a wavefront of wave_sort_kernel (one atomic
+  // per wavefront), or sorted here if it is too large or the queue is full
+  bool queue = big && n <= kWaveSortMax;
+  if (big) {
+    const unsigned long long m = __ballot(queue);
+    int base = 0;
+    const int lead = __ffsll((long long)m) - 1;
+    if (queue && (int)__lane_id() == lead)
+      base = atomicAdd(big, __popcll(m));
+    base = __shfl(base, lead < 0 ? 0 : lead);
+    const int at = base + __popcll(m & lanemask_lt());
+    if (queue && at < big_cap)
+      big[1 + at] = cell;
+    else
+      queue = false;
+  }
+  if (!queue) heap_sort(cptcls + b, n);
+}
+
+// One queued cell per wavefront: its ids copied to LDS (padded to a multiple
+// of four with INT_MAX), then every id is stored at its rank, the number of
+// smaller ids in the cell (ids are distinct).  O(n^2 / 64) LDS reads per
+// cell, all lanes reading the same four-id word at a time.
+constexpr int kSortWaves = 4;
+__global__ void __launch_bounds__(64 * kSortWaves)
+    wave_sort_kernel(const int *__restrict__ big, int big_cap,
+                     const int *__restrict__ cstart, int *cptcls) {
+  __shared__ int4 s_ids[kSortWaves][kWaveSortMax / 4];
+  const int lane = (int)__lane_id(), w = threadIdx.x >> 6;
+  int *ids = reinterpret_cast<int *>(s_ids[w]);
+  const int queued = min(big[0], big_cap);
+  for (int i = blockIdx.x * kSortWaves + w; i < queued;
+       i += gridDim.x * kSortWaves) {
+    const int cell = big[1 + i];
+    const int b = cstart[cell], n = cstart[cell + 1] - b;
+    const int n4 = (n + 3) >> 2;
+    for (int j = lane; j < 4 * n4; j += 64)
+      ids[j] = j < n ? cptcls[b + j] : INT_MAX;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int j = lane; j < n; j += 64) {
+      const int v = ids[j];
+      int rank = 0;
+      for (int k = 0; k < n4; ++k) {
+        const int4 q = s_ids[w][k];
+        rank += (q.x < v) + (q.y < v) + (q.z < v) + (q.w < v);
+      }
+      cptcls[b + rank] = v;
+    }
+    // every lane has read the cell before the next one overwrites it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -150,7 +205,7 @@ __global__ void __launch_bounds__(kScanThreads)
 // which spares a one-workgroup pass between the two kernels.
 __global__ void __launch_bounds__(kScanThreads)
     scan_apply_kernel(const int *in, int n, const int *__restrict__ tile_sum,
-                      int *out, int *clear) {
+                      int *out, int *clear, int *zero) {
   __shared__ int s_w[17];
   int pre = 0;
   for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads)
@@ -173,7 +228,10 @@ __global__ void __launch_bounds__(kScanThreads)
       if (clear) clear[base + k] = 0;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[0] = 0;
+    if (zero) *zero = 0;
+  }
 }
 
 __global__ void colouring_kernel(int nrows, int ncols, int *ccidx, int o1,
@@ -194,13 +252,14 @@ namespace {
 int scan_tiles(int n) { return (int)(((long long)n + kScanTile - 1) / kScanTile); }
 
 int scan_launch(const int *in, int *out, int n, ScanWork &w, hipStream_t st,
-                int *clear) {
+                int *clear, int *zero = nullptr) {
   if (n <= 0) return AMSC_OK;
   if (int rc = scan_reserve(w, n)) return rc;
   const int nt = scan_tiles(n);
   int *ts = static_cast<int *>(w.tmp);
   scan_sums_kernel<<<nt, kScanThreads, 0, st>>>(in, n, ts);
-  scan_apply_kernel<<<nt, kScanThreads, 0, st>>>(in, n, ts, out, clear);
+  scan_apply_kernel<<<nt, kScanThreads, 0, st>>>(in, n, ts, out, clear,
+                                                 zero);
   AMSC_LAUNCH_CHECK();
   return AMSC_OK;
 }
@@ -272,13 +331,19 @@ int launch_finish_binning(Binning &b, int np, const int *key,
                           hipStream_t st, const int *np_dev) {
   const int nc = b.nrows * b.ncols;
   if (active) return fail(AMSC_ERR_INVALID, "binning: unsupported gating");
-  if (int rc = scan_launch(b.count, b.cstart, nc, b.scan, st, b.count))
+  if (nc <= 0) return AMSC_OK;
+  if (int rc = scan_launch(b.count, b.cstart, nc, b.scan, st, b.count, b.big))
     return rc;
   if (np > 0)
     place_kernel<<<grid_for(np, 256), 256, 0, st>>>(np, np_dev, key, slot,
                                                     b.cstart, cptcls, active);
   cell_sort_kernel<<<grid_for(nc, 256), 256, 0, st>>>(nc, b.cstart, cptcls,
-                                                      active);
+                                                      active, b.big, b.big_cap);
+  if (b.big)
+    wave_sort_kernel<<<std::min<unsigned>(512, grid_for(b.big_cap,
+                                                        kSortWaves)),
+                       64 * kSortWaves, 0, st>>>(b.big, b.big_cap, b.cstart,
+                                                 cptcls);
   AMSC_LAUNCH_CHECK();
   return AMSC_OK;
 }

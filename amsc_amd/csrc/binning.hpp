@@ -30,8 +30,17 @@ struct Binning {
   int nrows = 0, ncols = 0;
   int *count = nullptr;   // [nc] cell counts (kept zero between uses)
   int *cstart = nullptr;  // [nc+1], written by each binning
+  // Crowded cells (9..kWaveSortMax particles) queued for a sort by a whole
+  // wavefront: big[0] = how many, big[1..big_cap] the cells.  Null: every
+  // cell is sorted by one thread.  Cleared by the scan of each binning.
+  int *big = nullptr;
+  int big_cap = 0;
   ScanWork scan;
 };
+
+// Largest cell sorted by one wavefront from LDS (larger: one thread, heap
+// sort in place).
+constexpr int kWaveSortMax = 1024;
 
 int binning_alloc(Binning &b, int nrows, int ncols);
 void binning_free(Binning &b);

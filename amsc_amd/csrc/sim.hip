@@ -984,6 +984,10 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   s->bin.ncols = 1;
   if (int rc = dalloc(*s, &s->bin.count, s->nc)) return rc;
   if (int rc = dalloc(*s, &s->bin.cstart, (size_t)s->nc + 1)) return rc;
+  // a cell of 9 or more particles is queued for a wavefront sort: at most
+  // cap / 9 of them
+  s->bin.big_cap = (int)(s->cap / 9 + 1);
+  if (int rc = dalloc(*s, &s->bin.big, (size_t)s->bin.big_cap + 1)) return rc;
   for (ScanWork *w : {&s->bin.scan, &s->ms_scan}) {
     size_t b = 0;
     if (int rc = scan_bytes(w == &s->ms_scan ? s->cap : s->nc, &b)) return rc;
@@ -995,6 +999,7 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   }
   if (int rc = launch_fill_int(*s, s->bin.count, s->nc, 0)) return rc;
   if (int rc = launch_fill_int(*s, s->bin.cstart, 1, 0)) return rc;
+  if (int rc = launch_fill_int(*s, s->bin.big, 1, 0)) return rc;
   for (int b = 0; b < 2; ++b) {
     P2GOut o;
     o.dzx = s->buf[b].f[PF_DZX];
@@ -1559,7 +1564,8 @@ long long bytes_estimate(int S, const amsc_params *prm, const int *np,
   bool elfs = false;
   for (int i = 0; i < S; ++i) elfs = elfs || prm[i].ms_mode == 1;
   // + upload staging: 3 FP64 values per initial particle
-  return cap * per_p + npt * 24 + (cap / 64 + 1) * 4 + nnt * per_n + nct * (elfs ? 24 : 8) +
+  return cap * per_p + npt * 24 + (cap / 64 + 1) * 4 + (cap / 9 + 2) * 4 +
+         nnt * per_n + nct * (elfs ? 24 : 8) +
          ntl * (6 * 8 + 4 * kTileCfgs) +
          S * (long long)(sizeof(Ctrl) + kTileCfgs * sizeof(Scen) + 40) +
          (1 << 16);  // allocations are rounded up to 256 bytes
