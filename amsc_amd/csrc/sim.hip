@@ -131,6 +131,11 @@ struct Sim {
   int *dense_d = nullptr;
   bool any_dense = false;
   P2GStaged staged{};
+  // queue of crowded cells for the binning's wavefront sort, handed to the
+  // binning (bin.big) once some cell has held more than 8 particles; until
+  // then every cell is sorted by one thread
+  int *big_q = nullptr;
+  bool wave_sort = false;
   long long particle_steps = 0;
   size_t bytes = 0;  // device memory requested (cached blocks may be larger)
   std::vector<std::pair<void *, size_t>> allocs;  // every dalloc block
@@ -690,7 +695,8 @@ int launch_graph(Sim &s, bool ms_step, int aux) {
                        (long long)s.tile_order << 7 |
                        (long long)(s.part_bs == 512) << 12 |
                        (long long)s.diag << 9 | (long long)s.mom_stored << 10 |
-                       (long long)s.aux_on << 11 | (long long)s.any_dense << 13;
+                       (long long)s.aux_on << 11 | (long long)s.any_dense << 13 |
+                       (long long)s.wave_sort << 14;
   auto it = s.graphs.find(gk);
   if (it == s.graphs.end()) {
     const int c0 = s.cur, k0 = s.kcur;
@@ -787,7 +793,13 @@ int launch_step(Sim &s) {
 // Flag for the dense-cell P2G the scenarios whose cells have held more than
 // kDensePPC particles (from the control blocks just read); a flag, once
 // set, stays.  The staged-value buffer is allocated with the first flag.
+// Likewise the binning's crowded-cell queue, from the first cell above 8.
 int update_dense(Sim &s) {
+  for (int i = 0; i < s.S && !s.wave_sort; ++i)
+    if (s.ctrl_host[i].max_ppc > 8) {
+      s.wave_sort = true;
+      s.bin.big = s.big_q;
+    }
   bool changed = false;
   for (int i = 0; i < s.S; ++i)
     if (!s.dense_h[i] && s.ctrl_host[i].max_ppc > kDensePPC) {
@@ -987,7 +999,7 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   // a cell of 9 or more particles is queued for a wavefront sort: at most
   // cap / 9 of them
   s->bin.big_cap = (int)(s->cap / 9 + 1);
-  if (int rc = dalloc(*s, &s->bin.big, (size_t)s->bin.big_cap + 1)) return rc;
+  if (int rc = dalloc(*s, &s->big_q, (size_t)s->bin.big_cap + 1)) return rc;
   for (ScanWork *w : {&s->bin.scan, &s->ms_scan}) {
     size_t b = 0;
     if (int rc = scan_bytes(w == &s->ms_scan ? s->cap : s->nc, &b)) return rc;
@@ -999,7 +1011,7 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   }
   if (int rc = launch_fill_int(*s, s->bin.count, s->nc, 0)) return rc;
   if (int rc = launch_fill_int(*s, s->bin.cstart, 1, 0)) return rc;
-  if (int rc = launch_fill_int(*s, s->bin.big, 1, 0)) return rc;
+  if (int rc = launch_fill_int(*s, s->big_q, 1, 0)) return rc;
   for (int b = 0; b < 2; ++b) {
     P2GOut o;
     o.dzx = s->buf[b].f[PF_DZX];

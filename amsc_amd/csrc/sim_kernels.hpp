@@ -331,9 +331,10 @@ __global__ void key_kernel(Tables T, const double *__restrict__ x,
   if (b0 >= np) return;
   const int ip = b0 + threadIdx.x;
   const bool valid = ip < np;
-  int k = 0;
+  int k = 0, sid = 0;
   if (valid) {
-    const Scen &S = T.scen[scen_of(T, ip)];
+    sid = scen_of(T, ip);
+    const Scen &S = T.scen[sid];
     k = S.coff + cell_of(x[ip], y[ip], S.P.nrows, S.P.ncols, S.P.hx, S.P.hy,
                          S.rc);
   }
@@ -341,8 +342,22 @@ __global__ void key_kernel(Tables T, const double *__restrict__ x,
   if (valid) {
     key[ip] = k;
     slot[ip] = sl;
-    Ctrl &C = T.ctrl[scen_of(T, ip)];
-    if (sl + 1 > C.max_ppc) atomicMax(&C.max_ppc, sl + 1);
+  }
+  // the largest cell count (slot + 1), raised in the control block past the
+  // value stored there: one atomic per wavefront when the wavefront's
+  // particles belong to one scenario, one per lane otherwise
+  const int s0 = __builtin_amdgcn_readfirstlane(sid);
+  if (__ballot(valid && sid != s0) == 0ull) {
+    int m = valid ? sl + 1 : 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int o = __shfl_xor(m, d);
+      m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m > T.ctrl[s0].max_ppc)
+      atomicMax(&T.ctrl[s0].max_ppc, m);
+  } else if (valid && sl + 1 > T.ctrl[sid].max_ppc) {
+    atomicMax(&T.ctrl[sid].max_ppc, sl + 1);
   }
 }
 

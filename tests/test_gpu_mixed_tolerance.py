@@ -7,15 +7,19 @@ path on the same inputs, to the stated tolerance:
   particle volumes spread to the nodes with the P2G weights) <= 1e-4, and
   the 99th percentile of the per-particle depth error (relative to the
   largest depth, particles matched by label) <= 1e-4;
-* an ensemble-shaped batch (48 scenarios as the 256-scenario ensemble:
+* the 256-scenario ensemble batch (the bench's scenarios, seeds 1000-1255:
   adaptive CFL 0.4, merge/split every 10 steps) after 20 steps, through two
   merge/split calls: nodal depth relative L2 <= 1e-4 for the median
-  scenario and <= 5e-3 for every scenario.
+  scenario, <= 1e-3 at the 90th percentile and <= 2e-2 for every scenario.
 
-Past that horizon the adaptive, friction-dominated flows amplify any
-per-step perturbation (the FP64 path itself departs by 1e-3 in L2 after 100
-steps when only its initial state is rounded to FP32), so the ensemble's
-error is reported by bench.py rather than bounded here."""
+The per-scenario bound is loose because some of these flows are badly
+conditioned: for the worst one (seed 1187) two FP64 runs whose initial
+depths, volumes, masses and areas differ by 1e-7 relative noise already
+differ by 8e-3 after 20 steps (tests/test_oracle_sensitivity.py), about
+what the mixed form shows there.  Past 20 steps the adaptive,
+friction-dominated flows amplify any per-step perturbation further, so
+the ensemble's error over its full run is reported by bench.py rather
+than bounded here."""
 import numpy as np
 import pytest
 
@@ -50,11 +54,11 @@ def test_mixed_within_tolerance_on_the_4m_workload():
     assert e["matched_frac"] > 0.999
 
 
-def test_mixed_within_tolerance_on_an_ensemble_batch():
+def test_mixed_within_tolerance_on_the_ensemble_batch():
     import bench
-    print("[mixed] ensemble batch, 20 steps", flush=True)
+    print("[mixed] 256-scenario ensemble batch, 20 steps", flush=True)
     cases = []
-    for i in range(48):
+    for i in range(256):
         dd = scenarios.ensemble_scenario(1000 + i)
         p = params_from_data(dd, max_steps=10**6)
         assert p["ms_on"] and p["ms_interval"] == 10 and p["dt_fixed"] == 0.0
@@ -63,6 +67,9 @@ def test_mixed_within_tolerance_on_an_ensemble_batch():
     m = run(cases, "mixed", 20)
     l2 = np.array([bench.field_errors(m[k], f[k], cases[k][0])["node_depth_rel_l2"]
                    for k in range(len(cases))])
-    print(f"[mixed] ensemble L2: median {np.median(l2):.3g}, max {l2.max():.3g}", flush=True)
+    print(f"[mixed] ensemble L2: median {np.median(l2):.3g}, "
+          f"p90 {np.percentile(l2, 90):.3g}, max {l2.max():.3g} "
+          f"(scenario {int(l2.argmax())})", flush=True)
     assert np.median(l2) <= 1e-4
-    assert l2.max() <= 5e-3
+    assert np.percentile(l2, 90) <= 1e-3
+    assert l2.max() <= 2e-2
