@@ -55,8 +55,8 @@ static const char *kNFName[NF_COUNT] = {
     "vvy"};
 
 // P2G tile shapes (node columns x node rows, one thread per node), tile
-// orders (XCD-grouped or linear) and particle-kernel workgroup sizes the
-// autotuner chooses from.  A P2G variant v is shape v % 3, order v / 3.
+// orders (descending column-major or linear) and particle-kernel workgroup
+// sizes the autotuner chooses from.  A P2G variant v is shape v % 3, order v / 3.
 constexpr int kTileCfgs = 3;
 constexpr int kP2GVariants = 2 * kTileCfgs;
 constexpr int kTileTC[kTileCfgs] = {16, 16, 8};
@@ -542,16 +542,16 @@ int merge_split(Sim &s, bool capture) {
   return rebin(s, capture);
 }
 
-template <typename R, bool XCD>
+template <typename R, bool DESC>
 int launch_p2g_order(Sim &s, const P2GArgs &a) {
   const dim3 g(s.ntiles[s.tile_cfg]);
   switch (s.tile_cfg) {
   case 0:
-    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 32, XCD>, g, dim3(512), a);
+    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 32, DESC>, g, dim3(512), a);
   case 1:
-    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 16, XCD>, g, dim3(256), a);
+    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 16, DESC>, g, dim3(256), a);
   default:
-    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 8, 32, XCD>, g, dim3(256), a);
+    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 8, 32, DESC>, g, dim3(256), a);
   }
 }
 

@@ -507,19 +507,18 @@ __device__ __forceinline__ int lds_slot(int i) { return i + (i >> 4); }
 // position, built cell-parallel) turns a position into its particle id with
 // one LDS read.  Each node thread folds its four cells in ascending cell id,
 // each cell in ascending particle id: the serial p2g order, so the sums are
-// the oracle's bit for bit.  Workgroups are mapped to tiles so that each of
-// the 8 XCDs (workgroup i runs on XCD i mod 8) gets a contiguous range of
-// tiles, whose shared halo cells then hit the same L2.
-template <typename R, int TC, int TR, bool XCD>
+// the oracle's bit for bit.  Tile order: linear (row of tiles by row of
+// tiles), or with DESC column-major from the last tile column down: particles
+// are stored about in column-major cell order, so the particles the
+// particle kernel wrote last (still in the memory-side cache) are staged
+// first, and the first ones, which the next particle kernel reads first,
+// last.
+template <typename R, int TC, int TR, bool DESC>
 __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   constexpr int NT = TC * TR, CH = NT, CHP = CH + CH / 16, SEG = TC + 1;
   static_assert(TR + 1 < 256 && SEG < 256, "owner map packs (s, j) in 16 bits");
   const int nb = gridDim.x, b = blockIdx.x;
-  int tile = b;
-  if (XCD) {
-    const int xcd = b & 7, q8 = nb >> 3, rem = nb & 7;
-    tile = xcd * q8 + (xcd < rem ? xcd : rem) + (b >> 3);
-  }
+  const int tile = DESC ? nb - 1 - b : b;
   // workgroup-uniform, said explicitly
   const int sid = __builtin_amdgcn_readfirstlane(a.tscen[tile]);
   const Scen &sc = a.T.scen[sid];
@@ -540,7 +539,13 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   using A = Arith<R>;
   const RecipT<A> rcr = recip_of<A>(sc);
   const int lt = tile - toff;
-  const int c0 = (lt % ntx) * TC, r0 = (lt / ntx) * TR;
+  int tcx = lt % ntx, tcy = lt / ntx;
+  if (DESC) {
+    const int nty = sc.ntiles / ntx;
+    tcx = lt / nty;
+    tcy = lt % nty;
+  }
+  const int c0 = tcx * TC, r0 = tcy * TR;
   const int t = threadIdx.x;
   const int lr = t % TR, lc = t / TR;
   const int c = c0 + lc, r = r0 + lr;
