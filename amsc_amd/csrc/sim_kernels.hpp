@@ -570,6 +570,22 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
     const int s = i / (nr + 1), j = i % (nr + 1);
     s_cs[s][j] = cstart[(ccl + s) * nrows + rl + j];
   }
+  // A tile whose cells hold no particle has nothing to stage or fold, and
+  // its nodes are written only on output steps: every wavefront tests the
+  // window's column ranges itself (the loads above have fetched the same
+  // words) and the whole workgroup leaves together, with zero partials.
+  if (a.aux < AUX_FULL) {
+    const int lane = t & 63;
+    int n = 0;
+    if (lane < nseg) {
+      const int base = (ccl + lane) * nrows;
+      n = cstart[base + rh + 1] - cstart[base + rl];
+    }
+    if (__ballot(n != 0) == 0ull) {
+      if (a.partials && t < 6) a.partials[tile * 6 + t] = 0.0;
+      return;
+    }
+  }
   __syncthreads();
   if (t == 0) {
     int o = 0;
