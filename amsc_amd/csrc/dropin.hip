@@ -159,6 +159,19 @@ __global__ void __launch_bounds__(kTNT)
   const int nr = rh - rl + 1, ncl = (cch - ccl + 1) * nr;
   const Recip R = recip_dev(hx, hy);
 
+  // a tile whose cells hold no particle adds nothing to its nodes: every
+  // wavefront tests the box's column ranges itself and the workgroup
+  // leaves together, before any barrier
+  {
+    const int lane = t & 63;
+    int np = 0;
+    if (lane <= cch - ccl) {
+      const int col0 = (ccl + lane) * nrows;
+      np = cstart[col0 + rh + 1] - cstart[col0 + rl];
+    }
+    if (__ballot(np != 0) == 0ull) return;
+  }
+
   // cell table: thread t holds box cells 2t and 2t+1 (column-major order)
   int cnt[2] = {0, 0}, col[2] = {0xFF, 0xFF};
 #pragma unroll

@@ -321,6 +321,28 @@ def test_crowded_tiles_bitwise():
     assert_same(o, g, "step 12")
 
 
+def test_pile_up_beyond_the_wavefront_sort():
+    """1500 particles in one cell and 600 in its neighbour: from the second
+    step call on the binning queues cells above 8 particles for a
+    wavefront's rank sort from LDS (the 600), and sorts the cells above
+    1024 (the 1500) on one thread; bit for bit either way."""
+    print("[sim] pile-up beyond the wavefront sort", flush=True)
+    rng = np.random.default_rng(11)
+    d = small_slope_case(nrows=40, ncols=40, hx=0.25, hy=0.25)
+    n1, n2 = 1500, 600
+    x = np.concatenate([rng.uniform(2.51, 2.74, n1), rng.uniform(2.76, 2.99, n2)])
+    y = np.concatenate([rng.uniform(2.51, 2.74, n1), rng.uniform(2.51, 2.74, n2)])
+    n = n1 + n2
+    hp = rng.uniform(0.2, 1.0, n)
+    Ap = np.full(n, 0.25 * 0.25 / 1500)
+    d.update(x=x, y=y, hp=hp, Ap=Ap, Vp=hp * Ap, Mp=2000.0 * hp * Ap,
+             vpx=np.zeros(n), vpy=np.zeros(n), mom_px=np.zeros(n),
+             mom_py=np.zeros(n), DT_FIXED=2e-4, NSTEPS=8)
+    o, g, _ = pair(d)
+    advance(o, g, 4)
+    assert_same(o, g, "step 4")
+
+
 def test_large_landslide_properties():
     """The ~2M-particle slope landslide: bit for bit against the oracle after
     the first step and after the first merge/split step that splits
