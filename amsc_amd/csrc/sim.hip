@@ -1480,43 +1480,68 @@ int autotune(Sim &s, int steps, int *cfg_out, int *bs_out, int *used,
              double *variant_ms) {
   if (steps <= 0) return fail(AMSC_ERR_INVALID, "steps must be positive");
   const bool timing0 = s.timing;
-  s.timing = true;
   int total = 0;
-  auto trial = [&](const char *name, double *avg) -> int {
-    if (int rc = reset_times(s)) return rc;
+  // one step untimed first: a run's first step stages the stored friction
+  // force and the data's momentum, which later steps do not
+  {
     int d = 0;
-    if (int rc = step_n(s, steps, &d)) return rc;
+    if (int rc = step_n(s, 1, &d)) return rc;
     total += d;
-    return avg_time(s, name, avg);
+  }
+  s.timing = true;
+  // one step of each candidate per round, `steps` rounds: every candidate
+  // sees about the same stage of the flow
+  auto rounds = [&](int n, const char *name, auto set, double *avg) -> int {
+    std::vector<double> sum(n, 0.0);
+    std::vector<int> cnt(n, 0);
+    for (int r = 0; r < steps && !s.finished; ++r)
+      for (int v = 0; v < n && !s.finished; ++v) {
+        set(v);
+        if (int rc = reset_times(s)) return rc;
+        int d = 0;
+        if (int rc = step_n(s, 1, &d)) return rc;
+        total += d;
+        double a = 0.0;
+        if (int rc = avg_time(s, name, &a)) return rc;
+        if (a > 0.0) {
+          sum[v] += a;
+          ++cnt[v];
+        }
+      }
+    for (int v = 0; v < n; ++v) avg[v] = cnt[v] ? sum[v] / cnt[v] : 0.0;
+    return AMSC_OK;
   };
+  const int v0 = s.tile_cfg + kTileCfgs * s.tile_order;
+  double pv[kP2GVariants], bv_ms[2];
+  if (int rc = rounds(kP2GVariants, "p2g_nodal", [&](int v) {
+        s.tile_cfg = v % kTileCfgs;
+        s.tile_order = v / kTileCfgs;
+      }, pv))
+    return rc;
+  int bv = v0;
   double best = 1e300;
-  int bv = s.tile_cfg + kTileCfgs * s.tile_order;
-  for (int v = 0; v < kP2GVariants && !s.finished; ++v) {
-    s.tile_cfg = v % kTileCfgs;
-    s.tile_order = v / kTileCfgs;
-    double a = 0.0;
-    if (int rc = trial("p2g_nodal", &a)) return rc;
-    if (variant_ms) variant_ms[v] = a;
-    if (a > 0.0 && a < best) {
-      best = a;
+  for (int v = 0; v < kP2GVariants; ++v)
+    if (pv[v] > 0.0 && pv[v] < best) {
+      best = pv[v];
       bv = v;
     }
-  }
   s.tile_cfg = bv % kTileCfgs;
   s.tile_order = bv / kTileCfgs;
+  const int bs0 = s.part_bs;
+  if (int rc = rounds(2, "particle", [&](int i) { s.part_bs = kPartBS[i]; }, bv_ms))
+    return rc;
+  int bbs = bs0;
   best = 1e300;
-  int bbs = s.part_bs;
-  for (int i = 0; i < 2 && !s.finished; ++i) {
-    s.part_bs = kPartBS[i];
-    double a = 0.0;
-    if (int rc = trial("particle", &a)) return rc;
-    if (variant_ms) variant_ms[kP2GVariants + i] = a;
-    if (a > 0.0 && a < best) {
-      best = a;
+  for (int i = 0; i < 2; ++i)
+    if (bv_ms[i] > 0.0 && bv_ms[i] < best) {
+      best = bv_ms[i];
       bbs = kPartBS[i];
     }
-  }
   s.part_bs = bbs;
+  if (variant_ms) {
+    for (int v = 0; v < kP2GVariants; ++v) variant_ms[v] = pv[v];
+    for (int i = 0; i < 2; ++i) variant_ms[kP2GVariants + i] = bv_ms[i];
+  }
   s.timing = timing0;
   (void)reset_times(s);
   if (cfg_out) *cfg_out = bv;

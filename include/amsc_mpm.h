@@ -372,11 +372,13 @@ int amsc_sim_set_graph(amsc_sim *s, int on);
  * not depend on either.  Default: variant 5 (8x32, linear) and 512. */
 #define AMSC_P2G_VARIANTS 6
 int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block);
-/* Autotuner: time every P2G variant, then every workgroup size, over
- * steps_per_variant live steps each and keep the fastest.  The steps are
- * real steps of the run; *steps_used (may be NULL) receives their number;
- * variant_ms (may be NULL, AMSC_P2G_VARIANTS + 2 entries) the average P2G
- * time of each variant, then the particle-kernel time of each size. */
+/* Autotuner: after one untimed step, time every P2G variant, then every
+ * workgroup size, over steps_per_variant live steps each (one step of each
+ * candidate per round) and keep the fastest.  The steps are real steps of
+ * the run; *steps_used (may be NULL) receives their number,
+ * 1 + steps_per_variant * (AMSC_P2G_VARIANTS + 2); variant_ms (may be
+ * NULL, AMSC_P2G_VARIANTS + 2 entries) the average P2G time of each
+ * variant, then the particle-kernel time of each size. */
 int amsc_sim_autotune(amsc_sim *s, int steps_per_variant, int *p2g_shape,
                       int *particle_block, int *steps_used,
                       double *variant_ms);

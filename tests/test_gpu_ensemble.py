@@ -154,7 +154,7 @@ def test_autotune_keeps_oracle_parity():
     d = small_slope_case()
     s = Simulation.from_data(d)
     shape, blk, used = s.autotune(2)
-    assert shape in range(6) and blk in (256, 512) and used == 16
+    assert shape in range(6) and blk in (256, 512) and used == 17
     o = oracle.OracleSim(params_from_data(d), particles_from_data(d), d["Z"])
     for _ in range(used + 3):
         o.step()
