@@ -59,8 +59,8 @@ static const char *kNFName[NF_COUNT] = {
 // sizes the autotuner chooses from.  A P2G variant v is shape v % 3, order v / 3.
 constexpr int kTileCfgs = 3;
 constexpr int kP2GVariants = 2 * kTileCfgs;
-constexpr int kTileTC[kTileCfgs] = {16, 16, 8};
-constexpr int kTileTR[kTileCfgs] = {32, 16, 32};
+constexpr int kTileTC[kTileCfgs] = {16, 8, 8};
+constexpr int kTileTR[kTileCfgs] = {32, 31, 32};
 constexpr int kPartBS[2] = {256, 512};
 // A scenario whose cells have held more than this many particles runs its
 // P2G on the dense-cell kernels (at 4 particles per cell the tiled P2G is
@@ -549,7 +549,7 @@ int launch_p2g_order(Sim &s, const P2GArgs &a) {
   case 0:
     return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 32, DESC>, g, dim3(512), a);
   case 1:
-    return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 16, DESC>, g, dim3(256), a);
+    return launch(s, "p2g_nodal", p2g_sweep_kernel<R, 8, 31, DESC>, g, dim3(64), a);
   default:
     return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 8, 32, DESC>, g, dim3(256), a);
   }
@@ -566,7 +566,7 @@ int launch_p2g_dense(Sim &s, const P2GArgs &a, int npos) {
   case 0:
     return launch(s, "p2g_dense", p2g_dense_node_kernel<R, 16, 32>, g, dim3(512), a);
   case 1:
-    return launch(s, "p2g_dense", p2g_dense_node_kernel<R, 16, 16>, g, dim3(256), a);
+    return launch(s, "p2g_dense", p2g_dense_node_kernel<R, 8, 31>, g, dim3(256), a);
   default:
     return launch(s, "p2g_dense", p2g_dense_node_kernel<R, 8, 32>, g, dim3(256), a);
   }

@@ -792,6 +792,348 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   }
 }
 
+// ------------------------------------------------------------- swept P2G
+// P2G + nodal update for one tile of W node columns x SR node rows, by one
+// wavefront that sweeps the tile's cell columns from left to right.
+//
+// Node (c, r) adds the particles of its cells (c-1, r-1), (c-1, r), (c, r-1),
+// (c, r) in that order: cell column c-1 rows r-1..r, then cell column c rows
+// r-1..r, each a contiguous CSR range.  Two lanes per node row hold the two
+// node columns beside the current cell column j: one folds node (r, j), which
+// then has all four of its cells and is finished, the other starts node
+// (r, j+1); the roles swap from column to column.  A lane walks the CSR range
+// of cells (j, r-1) and (j, r) in order, so every node adds its particles in
+// ascending cell, ascending id order: the serial sums of the tiled kernel,
+// bit for bit.  Each column's particles are staged through LDS once (up to
+// CAP positions a round); at four particles per cell every lane then folds
+// eight particles per column, so the whole wavefront is busy in every column
+// (a chunk of the tiled kernel keeps only the nodes beside its one or two
+// cell columns busy).  The wavefront is the workgroup, so no barrier waits
+// for another wavefront.  Pipeline: the fields of column j+1 load while
+// column j is folded, its particle ids (and the cell row of each binned
+// position) one column earlier, its CSR starts one column before that.
+struct SweepIds {
+  int ip[2], row[2];
+};
+
+// The cell row (0 .. nr-1) of binned position q in a column whose CSR starts
+// lane l holds in cs (l <= nr): the last row whose start is <= q.  A binary
+// search whose every round runs on every lane: the rounds read other lanes'
+// registers (ds_bpermute), which a lane that has left the loop would no
+// longer provide.  Six rounds cover nr <= 64.
+__device__ __forceinline__ int sweep_row_of(int cs, int q, int nr) {
+  int lo = 0, hi = nr;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int mid = (lo + hi) >> 1;
+    const int v = __shfl(cs, mid);
+    if (hi - lo > 1) {
+      if (v <= q) lo = mid;
+      else hi = mid;
+    }
+  }
+  return lo;
+}
+
+template <typename R, int W, int SR, bool DESC>
+__global__ void __launch_bounds__(64) p2g_sweep_kernel(P2GArgs a) {
+  constexpr int CAP = 128, CAPP = CAP + CAP / 4, ZR = SR + 2;
+  static_assert(2 * SR <= 64 && SR + 2 <= 64 && W + 1 <= 64, "tile too large");
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int tile = DESC ? nb - 1 - b : b;
+  const int sid = __builtin_amdgcn_readfirstlane(a.tscen[tile]);
+  const Scen &sc = a.T.scen[sid];
+  const KParams Pv = sc.P;
+  Ctrl *ctrl = &a.T.ctrl[sid];
+  if (!ctrl->active || (a.dense && a.dense[sid])) return;
+  const bool fric_read = ctrl->fric_read || ms_due(sc, *ctrl);
+  const int nrows = sc.P.nrows, ncols = sc.P.ncols, bc = sc.P.bc;
+  const double hx = sc.P.hx, hy = sc.P.hy, grav = sc.P.g, rho = sc.P.rho;
+  const int coff = sc.coff, noff = sc.noff, toff = sc.toff, ntx = sc.ntx;
+  const Recip rc = sc.rc;
+  using A = Arith<R>;
+  const RecipT<A> rcr = recip_of<A>(sc);
+  const int lt = tile - toff;
+  int tcx = lt % ntx, tcy = lt / ntx;
+  if (DESC) {
+    const int nty = sc.ntiles / ntx;
+    tcx = lt / nty;
+    tcy = lt % nty;
+  }
+  const int c0 = tcx * W, r0 = tcy * SR;
+  const int lane = threadIdx.x, row = lane >> 1, side = lane & 1;
+  const int r = r0 + row;  // this lane's node row
+  const bool row_ok = row < SR && r <= nrows;
+  const int *cstart = a.cstart + coff;
+  // cell rows rl..rh of every column of the window, cell columns ccl..cch
+  const int rl = r0 - 1 < 0 ? 0 : r0 - 1;
+  const int rh = r0 + SR - 1 > nrows - 1 ? nrows - 1 : r0 + SR - 1;
+  const int nr = rh - rl + 1;
+  const int ccl = c0 - 1 < 0 ? 0 : c0 - 1;
+  const int cch = c0 + W - 1 > ncols - 1 ? ncols - 1 : c0 + W - 1;
+
+  __shared__ double s_w[4][CAPP];  // xi, eta, mom x, mom y (FP64)
+  __shared__ R s_n[9][CAPP];       // Mp, Fpx, Fpy, frx, fry, F11, F12, F22, Vp
+  __shared__ double s_Z[(W + 2) * ZR];
+
+  // An empty window: nothing to stage or fold (its nodes are written on
+  // output steps only).  qsafe: a binned position of this tile, where loads
+  // for positions past a column's end are pointed (every load is issued
+  // whether or not its position exists, see the tiled kernel).
+  int qsafe = -1;
+  {
+    int n = 0, q0 = 0;
+    if (lane < cch - ccl + 1) {
+      const int base = (ccl + lane) * nrows;
+      q0 = cstart[base + rl];
+      n = cstart[base + rh + 1] - q0;
+    }
+    const unsigned long long m = __ballot(n != 0);
+    if (m == 0ull) {
+      if (a.aux < AUX_FULL) {
+        if (a.partials && lane < 6) a.partials[tile * 6 + lane] = 0.0;
+        return;
+      }
+    } else {
+      qsafe = __shfl(q0, __builtin_ctzll(m));
+    }
+  }
+  const bool any = qsafe >= 0;
+
+  // CSR starts of cell column j, rows rl..rh+1, one per lane (0 past them)
+  auto load_cs = [&](int j) -> int {
+    return (j >= 0 && j < ncols && lane <= nr) ? cstart[j * nrows + rl + lane] : 0;
+  };
+  // particle ids and cell rows of the first CAP positions of a column
+  auto fetch_ids = [&](int cs, SweepIds &I) {
+    const int B = __shfl(cs, 0), E = __shfl(cs, nr);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = B + h * 64 + lane;
+      const int qc = q < E ? q : (E > B ? E - 1 : qsafe);
+      I.ip[h] = a.cptcls[qc];
+      I.row[h] = sweep_row_of(cs, q, nr);
+    }
+  };
+  auto fetch_fields = [&](const SweepIds &I, P2GRaw<R> *Wf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) p2g_fields<R>(a, I.ip[h], fric_read ? I.ip[h] : 0, Wf[h]);
+  };
+
+  double pm = 0, ppx = 0, ppy = 0;  // particle sums (owned cells)
+  double nm = 0, nmx = 0, nmy = 0;  // sums of the finished nodes
+  double Mv = 0, mvx = 0, mvy = 0;
+  A FPx = 0, FPy = 0, Frx = 0, Fry = 0, Fix = 0, Fiy = 0;
+
+  // stage one particle (position q, binned cell (j, pr)) into LDS slot ls
+  auto stage_one = [&](const P2GRaw<R> &Wp, int ip, int j, int pr, int ls) {
+    const Local Lc = local_coords(Wp.x, Wp.y, j, pr, hx, hy, rc);
+    const int zb = (j - (c0 - 1)) * ZR + (pr - (r0 - 1));
+    const double z0 = s_Z[zb], z1 = s_Z[zb + 1];
+    const double z2 = s_Z[zb + ZR], z3 = s_Z[zb + ZR + 1];
+    // grad Z at the particle (g2pd(Z), optim_par.cpp:387)
+    double dzx = 0.0, dzy = 0.0;
+    dzx += shape_dx(Lc, 0, rc) * z0;
+    dzy += shape_dy(Lc, 0, rc) * z0;
+    dzx += shape_dx(Lc, 1, rc) * z1;
+    dzy += shape_dy(Lc, 1, rc) * z1;
+    dzx += shape_dx(Lc, 2, rc) * z2;
+    dzy += shape_dy(Lc, 2, rc) * z2;
+    dzx += shape_dx(Lc, 3, rc) * z3;
+    dzy += shape_dy(Lc, 3, rc) * z3;
+    // gravity along the slope with the well-balance factor,
+    // optim_par.cpp:450-453
+    const double wb = Wp.hp > 1e-10 ? 1.0 - (Wp.H / Wp.hp) : 1.0;
+    const double Vpd = Wp.Vp;
+    const double Fpx = -grav * Vpd * rho * dzx * wb;
+    const double Fpy = -grav * Vpd * rho * dzy * wb;
+    // mom_p = vpx * Mp (optim_par.cpp:600-601), DATA momentum at step 1
+    const double Mpd = Wp.Mp;
+    const double mx = a.mom_stored ? (double)Wp.m1 : (double)Wp.m1 * Mpd;
+    const double my = a.mom_stored ? (double)Wp.m2 : (double)Wp.m2 * Mpd;
+    s_w[0][ls] = Lc.xi;
+    s_w[1][ls] = Lc.eta;
+    s_w[2][ls] = mx;
+    s_w[3][ls] = my;
+    s_n[0][ls] = Wp.Mp;
+    s_n[1][ls] = (R)Fpx;
+    s_n[2][ls] = (R)Fpy;
+    R frx = Wp.frx, fry = Wp.fry;
+    if (!fric_read) {
+      A fbx, fby, ap;
+      voellmy<A>(Pv, (A)Wp.m1, (A)Wp.m2, Wp.hp, (A)Wp.Vp, fbx, fby, ap);
+      frx = ap * fbx;
+      fry = ap * fby;
+    }
+    s_n[3][ls] = frx;
+    s_n[4][ls] = fry;
+    s_n[5][ls] = Wp.f11;
+    s_n[6][ls] = Wp.f12;
+    s_n[7][ls] = Wp.f22;
+    s_n[8][ls] = Wp.Vp;
+    const bool owned = j >= c0 && j < c0 + W && pr >= r0 && pr < r0 + SR;
+    if (owned) {
+      pm += Mpd;
+      ppx += mx;
+      ppy += my;
+      if (a.aux >= AUX_FULL) {
+        const P2GOut *o = a.out;
+        global_ptr<R>(o->dzx)[ip] = (R)dzx;
+        global_ptr<R>(o->dzy)[ip] = (R)dzy;
+        global_ptr<R>(o->fpx)[ip] = (R)Fpx;
+        global_ptr<R>(o->fpy)[ip] = (R)Fpy;
+        global_ptr<R>(o->momx)[ip] = (R)mx;
+        global_ptr<R>(o->momy)[ip] = (R)my;
+      }
+    }
+  };
+  // fold the staged positions [q0, q1) (LDS slots from position base) into
+  // this lane's node; positions below mid are in the upper cell (j, r-1)
+  auto fold = [&](int q0, int q1, int base, int mid, bool right) {
+    for (int q = q0; q < q1; ++q) {
+      const int i = q - base + ((q - base) >> 2);
+      const bool upper = q < mid;
+      const double xi = s_w[0][i], eta = s_w[1][i];
+      // the basis of the node at corner (right ? 2 : 0) + (upper ? 1 : 0)
+      // of the particle's cell, as shape_t / shape_dx_t / shape_dy_t
+      const double fa = right ? xi : (1.0 - xi);
+      const double fb = upper ? eta : (1.0 - eta);
+      const double N = fa * fb;
+      Mv += N * (double)s_n[0][i];
+      mvx += N * s_w[2][i];
+      mvy += N * s_w[3][i];
+      const A Nr = (A)N;
+      FPx += Nr * (A)s_n[1][i];
+      FPy += Nr * (A)s_n[2][i];
+      Frx += Nr * (A)s_n[3][i];
+      Fry += Nr * (A)s_n[4][i];
+      const A xr = (A)xi, er = (A)eta;
+      const A Nx = (right ? rcr.ix : rcr.nix) * (upper ? er : (A(1) - er));
+      const A Ny = (right ? xr : (A(1) - xr)) * (upper ? rcr.iy : rcr.niy);
+      const A vp = s_n[8][i];
+      Fix += (Nx * (A)s_n[5][i] + Ny * (A)s_n[6][i]) * vp;
+      Fiy += (Nx * (A)s_n[6][i] + Ny * (A)s_n[7][i]) * vp;
+    }
+  };
+
+  if (any)
+    for (int i = lane; i < (W + 2) * ZR; i += 64) {
+      const int zc = c0 - 1 + i / ZR, zr = r0 - 1 + i % ZR;
+      s_Z[i] = (zc >= 0 && zc <= ncols && zr >= 0 && zr <= nrows)
+                   ? a.Z[noff + zr + zc * (nrows + 1)]
+                   : 0.0;
+    }
+
+  // pipeline: column of step k in csA/IA/WA, step k+1 in csB/IB, k+2 in csC
+  int csA = 0, csB = 0, csC = 0;
+  SweepIds IA, IB;
+  P2GRaw<R> WA[2];
+  if (any) {
+    csA = load_cs(c0 - 1);
+    csB = load_cs(c0);
+    fetch_ids(csA, IA);
+    fetch_ids(csB, IB);
+    fetch_fields(IA, WA);
+    csC = load_cs(c0 + 1);
+  }
+  __syncthreads();
+
+  for (int k = 0; k <= W; ++k) {
+    const int j = c0 - 1 + k;  // cell column of this step
+    // this lane's node: column j+1 (started here, corners 2/3) or column j
+    // (finished here, corners 0/1)
+    const bool right = side == (k & 1);
+    const int c = right ? j + 1 : j;
+    const bool node_ok = row_ok && c >= c0 && c < c0 + W && c <= ncols;
+    if (any) {
+      const int B = __shfl(csA, 0), E = __shfl(csA, nr);
+      const int n1 = E - B < CAP ? E - B : CAP;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (h * 64 + lane < n1) {
+          const int ls = h * 64 + lane + ((h * 64 + lane) >> 2);
+          stage_one(WA[h], IA.ip[h], j, rl + IA.row[h], ls);
+        }
+      // in flight during the fold: fields of step k+1, ids of step k+2,
+      // CSR starts of step k+3
+      P2GRaw<R> WB[2];
+      SweepIds IC;
+      fetch_fields(IB, WB);
+      fetch_ids(csC, IC);
+      const int csD = load_cs(j + 3);
+      __syncthreads();
+      // this lane's range: cells (j, r-1) and (j, r) where they exist
+      int lo = 0, mid = 0, hi = 0;
+      {
+        const int iu = r - 1 - rl < 0 ? 0 : r - 1 - rl;
+        const int im = r - rl < 0 ? 0 : (r - rl > nr ? nr : r - rl);
+        const int il = r + 1 - rl > nr ? nr : r + 1 - rl;
+        const int su = __shfl(csA, iu), sm = __shfl(csA, im), sl = __shfl(csA, il);
+        if (node_ok && j >= 0 && j < ncols) {
+          mid = sm;
+          lo = r >= 1 ? su : sm;
+          hi = r <= nrows - 1 ? sl : sm;
+        }
+      }
+      fold(lo > B ? lo : B, hi < B + n1 ? hi : B + n1, B, mid, right);
+      // columns of more than CAP positions: the rest, CAP at a time
+      for (int sub = B + CAP; sub < E; sub += CAP) {
+        __syncthreads();
+        const int ns = E - sub < CAP ? E - sub : CAP;
+        SweepIds IS;
+        P2GRaw<R> WS[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int q = sub + h * 64 + lane;
+          const int qc = q < E ? q : E - 1;
+          IS.ip[h] = a.cptcls[qc];
+          IS.row[h] = sweep_row_of(csA, q, nr);
+        }
+        fetch_fields(IS, WS);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (h * 64 + lane < ns) {
+            const int ls = h * 64 + lane + ((h * 64 + lane) >> 2);
+            stage_one(WS[h], IS.ip[h], j, rl + IS.row[h], ls);
+          }
+        __syncthreads();
+        fold(lo > sub ? lo : sub, hi < sub + ns ? hi : sub + ns, sub, mid, right);
+      }
+      csA = csB;
+      csB = csC;
+      csC = csD;
+      IA = IB;
+      IB = IC;
+      WA[0] = WB[0];
+      WA[1] = WB[1];
+    }
+    if (!right && node_ok) {
+      if (any || a.aux >= AUX_FULL)
+        p2g_node_update<R>(a, noff + r + c * (nrows + 1), c, r, nrows, ncols, bc,
+                           ctrl->dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry, Fix, Fiy);
+      nm += Mv;
+      nmx += mvx;
+      nmy += mvy;
+    }
+    if (!right) {
+      Mv = mvx = mvy = 0;
+      FPx = FPy = Frx = Fry = Fix = Fiy = 0;
+    }
+    __syncthreads();  // the next column's staging overwrites the LDS
+  }
+
+  if (a.partials) {
+    // conservation sums (optim_par.cpp:391-428), FP64: tile partials here,
+    // the scenario's reduction in conservation_kernel
+    const double v[6] = {pm, ppx, ppy, nm, nmx, nmy};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double sm = wave_sum(v[k]);
+      if (lane == k) a.partials[tile * 6 + k] = sm;
+    }
+  }
+}
+
 // ------------------------------------------------------------- dense P2G
 // P2G for scenarios whose cells hold many particles (a deposit piling up at
 // the foot of a slope).  The tiled P2G stages its tile's particles through
@@ -878,9 +1220,10 @@ __global__ void __launch_bounds__(256) p2g_stage_kernel(P2GArgs a) {
 // the node's four cells in ascending cell id, each cell's staged particles
 // in ascending id, then the nodal update.
 template <typename R, int TC, int TR>
-__global__ void __launch_bounds__(TC * TR)
+__global__ void __launch_bounds__((TC * TR + 63) / 64 * 64)
     __attribute__((amdgpu_waves_per_eu(1, 2))) p2g_dense_node_kernel(P2GArgs a) {
-  constexpr int NT = TC * TR, D = 8;
+  // NT nodes, launched as NB threads (whole wavefronts)
+  constexpr int NT = TC * TR, NB = (NT + 63) / 64 * 64, D = 8;
   using A = Arith<R>;
   const int tile = blockIdx.x;
   const int sid = __builtin_amdgcn_readfirstlane(a.tscen[tile]);
@@ -894,7 +1237,7 @@ __global__ void __launch_bounds__(TC * TR)
   const int c0 = (lt % sc.ntx) * TC, r0 = (lt / sc.ntx) * TR;
   const int t = threadIdx.x;
   const int c = c0 + t / TR, r = r0 + t % TR;
-  const bool node_ok = c <= ncols && r <= nrows;
+  const bool node_ok = t < NT && c <= ncols && r <= nrows;
   const int *cstart = a.cstart + sc.coff;
   const P2GRec<R> *recs = static_cast<const P2GRec<R> *>(a.st.rec);
   double pm = 0, ppx = 0, ppy = 0;  // particle sums (the node's own cell)
@@ -962,7 +1305,7 @@ __global__ void __launch_bounds__(TC * TR)
   }
   if (a.partials) {
     // conservation sums, FP64: tile partials as the tiled P2G writes them
-    __shared__ double s_red[6][NT / 64];
+    __shared__ double s_red[6][NB / 64];
     const int lane = t & 63, w = t >> 6;
     const double v[6] = {pm, ppx, ppy, Mv, mvx, mvy};
 #pragma unroll
@@ -973,7 +1316,7 @@ __global__ void __launch_bounds__(TC * TR)
     __syncthreads();
     if (t < 6) {
       double sm = 0.0;
-      for (int ww = 0; ww < NT / 64; ++ww) sm += s_red[t][ww];
+      for (int ww = 0; ww < NB / 64; ++ww) sm += s_red[t][ww];
       a.partials[tile * 6 + t] = sm;
     }
   }

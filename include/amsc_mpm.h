@@ -366,10 +366,11 @@ int amsc_sim_ms_counts(amsc_sim *s, int out[2]);
  * extents then follow the particle capacity; results are unchanged).
  * Ignored while per-kernel timing is on. */
 int amsc_sim_set_graph(amsc_sim *s, int on);
-/* P2G variant v in 0..5: tile shape v % 3 (0: 16x32 nodes, 1: 16x16,
- * 2: 8x32), tile order v / 3 (0: column-major from the last tile column
- * down, 1: linear); particle-kernel workgroup size 256 or 512.  Results do
- * not depend on either.  Default: variant 5 (8x32, linear) and 512. */
+/* P2G variant v in 0..5: tile shape v % 3 (0: 16x32 nodes, 1: 8x31 nodes
+ * swept by one wavefront column by column, 2: 8x32), tile order v / 3 (0:
+ * column-major from the last tile column down, 1: linear); particle-kernel
+ * workgroup size 256 or 512.  Results do not depend on either.  Default:
+ * variant 5 (8x32, linear) and 512. */
 #define AMSC_P2G_VARIANTS 6
 int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block);
 /* Autotuner: after one untimed step, time every P2G variant, then every

@@ -148,6 +148,33 @@ def test_tile_shapes_and_blocks_give_identical_results():
             s.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_p2g_variant_gives_the_same_batch(precision):
+    """Four scenarios of different grids and particle counts (merge/split in
+    one of them) advanced as one batch by each P2G variant (tiled and swept
+    kernels, both tile orders) and particle workgroup size: identical bits,
+    in both precisions."""
+    from amsc_amd.simulation import Ensemble
+    print(f"[ens] p2g variants {precision}", flush=True)
+    cases = mixed_cases()
+    cap = 8 * sum(len(p["x"]) for _, p, _ in cases)
+    ref = None
+    for shape in range(6):
+        for blk in (256, 512):
+            e = Ensemble(cases, precision, capacity=cap)
+            e.set_tiling(shape, blk)
+            e.step(12)
+            e.set_aux(True)
+            e.step(1)
+            got = {(k, f): e.get(k, f) for k in range(len(cases))
+                   for f in FIELDS + ("vvxL",)}
+            if ref is None:
+                ref = got
+            for key in got:
+                assert np.array_equal(bits(got[key]), bits(ref[key])), (shape, blk, key)
+            e.close()
+
+
 def test_autotune_keeps_oracle_parity():
     from amsc_amd.simulation import Simulation
     print("[ens] autotune", flush=True)
@@ -424,18 +451,21 @@ def test_empty_scenario_in_a_batch():
     s.close()
 
 
-def test_dense_scenario_in_a_batch():
+@pytest.mark.parametrize("p2g_variant", [5, 1])
+def test_dense_scenario_in_a_batch(p2g_variant):
     """A crowded scenario (about 200 particles per cell, so it runs the
-    dense-cell P2G after its first step) beside an ordinary one (tiled P2G)
-    in one batch, FP64 and mixed: each ends in the state of its own run
-    alone, bit for bit, and the FP64 batch in the oracle's."""
+    dense-cell P2G after its first step) beside an ordinary one (tiled or
+    swept P2G, with the dense kernels of that tile shape) in one batch, FP64
+    and mixed: each ends in the state of its own run alone, bit for bit, and
+    the FP64 batch in the oracle's."""
     from amsc_amd.simulation import Ensemble
     from test_gpu_sim import crowded_case
-    print("[ens] dense scenario in a batch", flush=True)
+    print(f"[ens] dense scenario in a batch, P2G variant {p2g_variant}", flush=True)
     cases = [scen(crowded_case(n=12000, seed=3)), scen(small_slope_case())]
     for precision in ("fp64", "mixed"):
         cap = 4 * sum(len(p["x"]) for _, p, _ in cases)
         e = Ensemble(cases, precision, capacity=cap)
+        e.set_tiling(p2g_variant, 512)
         e.step(1)
         e.set_timing(True)
         e.step(7)

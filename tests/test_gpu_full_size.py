@@ -1,6 +1,6 @@
 """The 4M-particle configuration at its full size (2048 x 2048 cells,
-hx = 0.5, fractal slope, polygonal release; amsc_amd.scenarios.large_fractal):
-bit for bit against the oracle after the first step and after the first
+hx = 0.5, fractal slope, polygonal release; amsc_amd.scenarios.large_fractal),
+with the tiled and the swept P2G: bit for bit against the oracle after the first step and after the first
 merge/split step that splits or merges particles (step 30; steps 10 and 20
 find nothing to do), with the same merge/split counts; P2G mass and
 momentum transfer to round-off every step
@@ -22,14 +22,18 @@ def bits(a):
     return np.ascontiguousarray(a, np.float64).view(np.int64)
 
 
-def test_large_fractal_4m():
+# P2G variants: 5 = 8x32 tiles in linear order (the default), 1 = the swept
+# kernel (one wavefront per 8x31 tile) in descending column-major order
+@pytest.mark.parametrize("p2g_variant", [5, 1])
+def test_large_fractal_4m(p2g_variant):
     from amsc_amd.simulation import Simulation
-    print("[full size] large fractal 4M", flush=True)
+    print(f"[full size] large fractal 4M, P2G variant {p2g_variant}", flush=True)
     d = scenarios.large_fractal(nsteps=40)
     prm = params_from_data(d)
     part = particles_from_data(d)
     assert len(part["x"]) >= 3_900_000
     g = Simulation(prm, part, d["Z"], capacity=3 * len(part["x"]))
+    g.set_tiling(p2g_variant, 512)
     o = oracle.OracleSim(prm, part, d["Z"])
     m0 = float(np.sum(d["Mp"]))
     worst = 0.0
