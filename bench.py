@@ -530,8 +530,11 @@ def main():
     state64 = grab(sim)
     sim.close()
 
+    # the P2G kernel the run used: variants 1 and 4 are the swept kernel,
+    # the others the tiled one (measured traffic per kernel from counters)
+    p2g_kernel = "p2g_sweep" if r64["variant"] % 3 == 1 else "p2g_nodal"
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_p2g_nodal_{args.config}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{p2g_kernel}_{args.config}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -594,7 +597,7 @@ def main():
                    "active_nodes": int(n_act),
                    "tiling": {"p2g_variant": r64["variant"],
                               "particle_block": r64["block"]}},
-        "roofline": {"bound": "hbm", "kernel": "p2g_nodal",
+        "roofline": {"bound": "hbm", "kernel": p2g_kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
