@@ -219,19 +219,34 @@ __global__ void __launch_bounds__(kScanThreads)
   load4(in, base, n, v);
   int total;
   const int ex = block_excl_scan(v[0] + v[1] + v[2] + v[3], s_w, &total);
-  int run = tile_off + ex;
+  // out[i] is the exclusive prefix at i (out[n] the sum of all): this
+  // thread's four values are out[base .. base+3], one 16-byte store where
+  // they are aligned and inside the array
+  int e[4];
+  e[0] = tile_off + ex;
+  e[1] = e[0] + v[0];
+  e[2] = e[1] + v[1];
+  e[3] = e[2] + v[2];
+  if (base + 3 <= n && (reinterpret_cast<uintptr_t>(out + base) & 15) == 0) {
+    *reinterpret_cast<int4 *>(out + base) = make_int4(e[0], e[1], e[2], e[3]);
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) {
-      run += v[k];
-      out[base + k + 1] = run;
-      if (clear) clear[base + k] = 0;
+    for (int k = 0; k < 4; ++k)
+      if (base + k <= n) out[base + k] = e[k];
+  }
+  // out[n] when n is a multiple of four falls past the last thread's values
+  if (base + 4 == n) out[n] = e[3] + v[3];
+  // counts that are not zero are cleared (most cells of a grid hold none)
+  if (clear && (v[0] | v[1] | v[2] | v[3]) != 0) {
+    if (base + 3 < n && (reinterpret_cast<uintptr_t>(clear + base) & 15) == 0) {
+      *reinterpret_cast<int4 *>(clear + base) = make_int4(0, 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (base + k < n) clear[base + k] = 0;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    out[0] = 0;
-    if (zero) *zero = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && zero) *zero = 0;
 }
 
 __global__ void colouring_kernel(int nrows, int ncols, int *ccidx, int o1,
