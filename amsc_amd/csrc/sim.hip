@@ -549,7 +549,10 @@ int launch_p2g_order(Sim &s, const P2GArgs &a) {
   case 0:
     return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 16, 32, DESC>, g, dim3(512), a);
   case 1:
-    return launch(s, "p2g_nodal", p2g_sweep_kernel<R, 8, 31, DESC>, g, dim3(64), a);
+    if constexpr (sizeof(R) == 4)
+      return launch(s, "p2g_nodal", p2g_sweep_mixed_kernel<8, 31, DESC>, g, dim3(64), a);
+    else
+      return launch(s, "p2g_nodal", p2g_sweep_kernel<R, 8, 31, DESC>, g, dim3(64), a);
   default:
     return launch(s, "p2g_nodal", p2g_nodal_kernel<R, 8, 32, DESC>, g, dim3(256), a);
   }

@@ -836,7 +836,7 @@ __device__ __forceinline__ int sweep_row_of(int cs, int q, int nr) {
 }
 
 template <typename R, int W, int SR, bool DESC>
-__global__ void __launch_bounds__(64) p2g_sweep_kernel(P2GArgs a) {
+__device__ __forceinline__ void p2g_sweep_body(P2GArgs a) {
   constexpr int CAP = 128, CAPP = CAP + CAP / 4, ZR = SR + 2;
   static_assert(2 * SR <= 64 && SR + 2 <= 64 && W + 1 <= 64, "tile too large");
   const int nb = gridDim.x, b = blockIdx.x;
@@ -874,6 +874,7 @@ __global__ void __launch_bounds__(64) p2g_sweep_kernel(P2GArgs a) {
 
   __shared__ double s_w[4][CAPP];  // xi, eta, mom x, mom y (FP64)
   __shared__ R s_n[9][CAPP];       // Mp, Fpx, Fpy, frx, fry, F11, F12, F22, Vp
+  __shared__ unsigned char s_own[CAP];  // cell row of each staged position
   __shared__ double s_Z[(W + 2) * ZR];
 
   // An empty window: nothing to stage or fold (its nodes are written on
@@ -904,16 +905,33 @@ __global__ void __launch_bounds__(64) p2g_sweep_kernel(P2GArgs a) {
   auto load_cs = [&](int j) -> int {
     return (j >= 0 && j < ncols && lane <= nr) ? cstart[j * nrows + rl + lane] : 0;
   };
-  // particle ids and cell rows of the first CAP positions of a column
+  // particle ids and cell rows of the first CAP positions of a column.  The
+  // cell row of a position: in the mixed form the lane of each row marks its
+  // positions in an owner map (LDS) that the lanes then read; in FP64 a
+  // binary search over the CSR starts (each measured the faster of the two
+  // in its precision on the 4M workload, 121 against 133 us and 170 against
+  // 189 us)
   auto fetch_ids = [&](int cs, SweepIds &I) {
     const int B = __shfl(cs, 0), E = __shfl(cs, nr);
+    if constexpr (sizeof(R) == 4) {
+      const int cs1 = __shfl_down(cs, 1);  // start of the next row
+      if (lane < nr) {
+        const int hi = cs1 < B + CAP ? cs1 : B + CAP;
+        for (int q = cs; q < hi; ++q) s_own[q - B] = (unsigned char)lane;
+      }
+      __syncthreads();
+    }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int q = B + h * 64 + lane;
       const int qc = q < E ? q : (E > B ? E - 1 : qsafe);
       I.ip[h] = a.cptcls[qc];
-      I.row[h] = sweep_row_of(cs, q, nr);
+      if constexpr (sizeof(R) == 4)
+        I.row[h] = q < E ? s_own[h * 64 + lane] : 0;
+      else
+        I.row[h] = sweep_row_of(cs, q, nr);
     }
+    if constexpr (sizeof(R) == 4) __syncthreads();  // the map is rewritten next
   };
   auto fetch_fields = [&](const SweepIds &I, P2GRaw<R> *Wf) {
 #pragma unroll
@@ -1132,6 +1150,19 @@ __global__ void __launch_bounds__(64) p2g_sweep_kernel(P2GArgs a) {
       if (lane == k) a.partials[tile * 6 + k] = sm;
     }
   }
+}
+
+// The FP64 kernel as the compiler allocates it (about 204 registers, two
+// wavefronts per SIMD); the mixed one held to three wavefronts per SIMD
+// (168 registers, a few spilled): the faster of each on the 4M workload.
+template <typename R, int W, int SR, bool DESC>
+__global__ void __launch_bounds__(64) p2g_sweep_kernel(P2GArgs a) {
+  p2g_sweep_body<R, W, SR, DESC>(a);
+}
+template <int W, int SR, bool DESC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+    p2g_sweep_mixed_kernel(P2GArgs a) {
+  p2g_sweep_body<float, W, SR, DESC>(a);
 }
 
 // ------------------------------------------------------------- dense P2G
