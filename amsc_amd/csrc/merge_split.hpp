@@ -55,23 +55,39 @@ int ms_decide(const MsParams &m, int np, int nc, const double *dt_dev,
               const int *cptcls, int *act, int *partner, int *cnt_dev,
               int *counts, hipStream_t st);
 
-// Action flag of one particle, merge_split_ops_cmes.h:143-172.
-template <typename R>
-__device__ __forceinline__ int ms_flag_one(const MsParams &m, double dt,
-                                           const MsIn<R> &in, int ip) {
-  const double r_i = sqrt((double)in.Ap[ip]);
-  const double exx = in.vxdx[ip], eyy = in.vydy[ip];
-  const double exy = 0.5 * ((double)in.vxdy[ip] + (double)in.vydx[ip]);
+// Action flag of one particle, merge_split_ops_cmes.h:143-172, in two
+// parts: the tests on its area and velocity gradients (as stored) and dt,
+// as bits (MS_BIT_STRAIN: stretched or sheared; MS_BIT_THICK: area below
+// the merge radius), then the tests on its depth and level.
+enum : int { MS_BIT_STRAIN = 1, MS_BIT_THICK = 2 };
+__device__ __forceinline__ int ms_strain_bits(const MsParams &m, double dt,
+                                              double Ap, double vxdx,
+                                              double vxdy, double vydx,
+                                              double vydy) {
+  const double r_i = sqrt(Ap);
+  const double exx = vxdx, eyy = vydy;
+  const double exy = 0.5 * (vxdy + vydx);
   const double gdot = sqrt(2.0 * (exx * exx + eyy * eyy + 2.0 * exy * exy));
   const bool stretched = r_i > m.r_split;
   const bool sheared = gdot * dt > m.shear_split;
   const bool thick = r_i < m.r_merge;
-  const bool thin_skip = in.hp[ip] < m.split_hp_min;
-  const int lvl = in.level[ip];
-  if ((stretched || sheared) && !thin_skip && lvl > m.min_level)
-    return MS_SPLIT;
-  if (thick && in.hp[ip] > m.hp_min && lvl < m.max_level) return MS_CANDIDATE;
+  return (stretched || sheared ? MS_BIT_STRAIN : 0) | (thick ? MS_BIT_THICK : 0);
+}
+__device__ __forceinline__ int ms_flag_of_bits(const MsParams &m, int bits,
+                                               double hp, int lvl) {
+  const bool thin_skip = hp < m.split_hp_min;
+  if ((bits & MS_BIT_STRAIN) && !thin_skip && lvl > m.min_level) return MS_SPLIT;
+  if ((bits & MS_BIT_THICK) && hp > m.hp_min && lvl < m.max_level)
+    return MS_CANDIDATE;
   return MS_KEEP;
+}
+template <typename R>
+__device__ __forceinline__ int ms_flag_one(const MsParams &m, double dt,
+                                           const MsIn<R> &in, int ip) {
+  const int bits = ms_strain_bits(m, dt, (double)in.Ap[ip], (double)in.vxdx[ip],
+                                  (double)in.vxdy[ip], (double)in.vydx[ip],
+                                  (double)in.vydy[ip]);
+  return ms_flag_of_bits(m, bits, in.hp[ip], in.level[ip]);
 }
 
 // Greedy pairing of one cell's candidates, ascending id

@@ -457,7 +457,7 @@ int merge_split(Sim &s, bool capture) {
       return rc;
   if (n > 0)
     if (int rc = launch(s, "ms_flag", ens_flag_kernel<R>, gp, b256, T, in, E,
-                        key, s.act, s.partner))
+                        key, (const int *)s.wts, s.act, s.partner))
       return rc;
   if (int rc = launch(s, "ms_pair", ens_pair_kernel<R>, dim3(grid_for(s.nc, 256)),
                       b256, T, (const int *)s.cell_off, s.nc, in,
@@ -466,7 +466,8 @@ int merge_split(Sim &s, bool capture) {
     return rc;
   if (n > 0)
     if (int rc = launch(s, "ms_refresh", ens_refresh_kernel<R>, gp, b256, T,
-                        in, E, key, s.act, s.partner, (const int *)s.ms_cnt))
+                        in, E, key, (const int *)s.wts, s.act, s.partner,
+                        (const int *)s.ms_cnt))
       return rc;
   if (int rc = launch(s, "ms_serial", ens_serial_kernel<R>, dim3(s.S), dim3(64),
                       T, in, (const int *)s.bin.cstart, (const int *)s.cptcls,
@@ -676,6 +677,7 @@ int step_body(Sim &s, bool ms_step, int aux, bool capture, bool *skipped) {
     a.avx = static_cast<const double *>(s.nod.f[NF_AVX]);
     a.avy = static_cast<const double *>(s.nod.f[NF_AVY]);
     a.Z = s.Z;
+    a.ms_bits = s.wts;  // read by the next step's merge/split
     const int rc = s.part_bs == 512 ? launch_particle_bs<R, 512>(s, a, aux, n)
                                     : launch_particle_bs<R, 256>(s, a, aux, n);
     if (rc) return rc;
@@ -1130,6 +1132,7 @@ int upload_ctrl(Sim *s, int first) {
     c.dt = 1.0e-3;  // gpu_offload.cpp:255
     c.active = 1;
     c.fric_set = 1;  // the first P2G reads Fric_p = Ap * 0
+    c.flags_it = -1;
   }
   if (c0.empty()) return AMSC_OK;
   AMSC_HIP(hipMemcpyAsync(s->ctrl + first, c0.data(), sizeof(Ctrl) * c0.size(),
@@ -1232,10 +1235,12 @@ int repack(Sim *o, Sim *s, int nkeep, const int *keep, int nadd,
     if (np_keep > 0)
       rc = s->mixed ? launch(*s, "repack", repack_particles_kernel<float>,
                              dim3(grid_for(np_keep, 256)), dim3(256),
-                             o->buf[o->cur], s->buf[0], mp)
+                             o->buf[o->cur], s->buf[0], mp,
+                             (const int *)o->wts, s->wts)
                     : launch(*s, "repack", repack_particles_kernel<double>,
                              dim3(grid_for(np_keep, 256)), dim3(256),
-                             o->buf[o->cur], s->buf[0], mp);
+                             o->buf[o->cur], s->buf[0], mp,
+                             (const int *)o->wts, s->wts);
     if (!rc)
       rc = s->mixed ? launch(*s, "repack", repack_nodes_kernel<float>,
                              dim3(grid_for(nn_keep, 256)), dim3(256), o->nod,
@@ -1398,6 +1403,23 @@ int set_field(Sim &s, int sc, const char *name, const double *in, int n) {
   const int p0 = s.poff_h[sc], np = s.poff_h[sc + 1] - p0;
   if (n != np) return fail(AMSC_ERR_INVALID, "length must equal num_particles");
   if (np == 0) return AMSC_OK;
+  if (k == PF_AP || k == PF_VXDX || k == PF_VXDY || k == PF_VYDX || k == PF_VYDY) {
+    // A step before a merge/split step of a fixed-dt run under the solvers'
+    // rule keeps these only where its strain tests passed and hands the
+    // tests on as bits; a new value could not reach them.
+    int v[2];
+    AMSC_HIP(hipMemcpyAsync(&v[0], &s.ctrl[sc].it, sizeof(int),
+                            hipMemcpyDeviceToHost, s.st));
+    AMSC_HIP(hipMemcpyAsync(&v[1], &s.ctrl[sc].flags_it, sizeof(int),
+                            hipMemcpyDeviceToHost, s.st));
+    AMSC_HIP(hipStreamSynchronize(s.st));
+    if (!s.ended[sc] && v[1] == v[0] + 1)
+      return fail(AMSC_ERR_STALE,
+                  std::string("field '") + name +
+                      "' is read by the next merge/split as the last step "
+                      "left it; set it with amsc_sim_set_aux enabled before "
+                      "that step");
+  }
   // the next P2G reads the stored friction force as it was before the write
   if (int rc = fric_materialize(s, sc)) return rc;
   {

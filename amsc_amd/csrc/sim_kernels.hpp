@@ -100,6 +100,9 @@ struct Ctrl {
   // doubles, by step parity, spread over 64 addresses so that the per-block
   // atomics of the particle kernel do not all hit one line
   unsigned long long cfl[2][64][3];
+  // the step whose merge/split strain bits the previous particle update
+  // wrote (see flags_due), or -1
+  int flags_it;
 };
 constexpr size_t kCtrlHead = 64;  // bytes before cfl (what the host reads)
 static_assert(offsetof(Ctrl, cfl) == kCtrlHead, "control block header size");
@@ -172,6 +175,19 @@ __device__ __forceinline__ bool ms_due(const Scen &S, const Ctrl &C) {
 // Which optional fields a step writes: none; what the next merge/split
 // reads (Ap, Fb, velocity gradients); every output field and the nodal set.
 enum : int { AUX_NONE = 0, AUX_MS = 1, AUX_FULL = 2 };
+
+// Whether the particle update of step `it` evaluates the strain tests of
+// the merge/split flag of step it + 1 itself (ms_strain_bits): the next step
+// merges and splits under the solvers' rule with a fixed dt (the tests need
+// that dt).  The update then stores the bits of every particle and writes
+// area, friction and velocity gradients only where a bit is set (no other
+// particle can split or merge; the friction force of every particle is still
+// written, for the next P2G), and the flag kernel finishes the flag from the
+// bits, the depth and the level.
+__device__ __forceinline__ bool flags_due(const Scen &S, int it) {
+  return S.ms.mode == 0 && S.P.dt_fixed > 0.0 && S.ms_on &&
+         S.ms_interval > 0 && (it + 1) % S.ms_interval == 0;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -1393,6 +1409,7 @@ struct PartArgs {
   PPtr p;
   const int *key_in;  // step-start cell of each particle
   int *key_out, *slot, *count;
+  int *ms_bits;  // merge/split strain bits for the next step (flags_due)
   const void *vvxL, *vvyL;  // type R
   const double *avx, *avy;
   const double *Z;
@@ -1403,9 +1420,9 @@ struct PartArgs {
 // new cell key of one particle of an active scenario.
 template <typename R, int AUX>
 __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
-                                              double dt, int ip, int &newkey,
-                                              double &vmx, double &vmy,
-                                              double &hmx) {
+                                              double dt, int it, int ip,
+                                              int &newkey, double &vmx,
+                                              double &vmy, double &hmx) {
   using A = Arith<R>;
   const KParams &P = sc.P;
   const PPtr &p = a.p;
@@ -1532,6 +1549,17 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
     // optim_par.cpp:456-459; other P2Gs recompute it
     fld<R>(p, PF_FRX)[ip] = Ap * Fbx;
     fld<R>(p, PF_FRY)[ip] = Ap * Fby;
+  }
+  bool ms_fields = AUX >= AUX_MS;
+  if (AUX == AUX_MS && flags_due(sc, it)) {
+    // the strain tests from the values as stored
+    const int bits = ms_strain_bits(sc.ms, P.dt_fixed, (double)(R)Ap,
+                                    (double)(R)vxdx, (double)(R)vxdy,
+                                    (double)(R)vydx, (double)(R)vydy);
+    a.ms_bits[ip] = bits;
+    ms_fields = bits != 0;
+  }
+  if (ms_fields) {
     fld<R>(p, PF_AP)[ip] = Ap;
     fld<R>(p, PF_FBX)[ip] = Fbx;
     fld<R>(p, PF_FBY)[ip] = Fby;
@@ -1609,7 +1637,9 @@ __global__ void __launch_bounds__(BS)
   double vmx = 0.0, vmy = 0.0, hmx = 0.0;
   if (valid) {
     if (act) {
-      particle_body<R, AUX>(a, T.scen[sid], C.dt, ip, newkey, vmx, vmy, hmx);
+      particle_body<R, AUX>(a, T.scen[sid], C.dt, C.it, ip, newkey, vmx, vmy, hmx);
+      if (AUX == AUX_MS && (EDGE || threadIdx.x == 0) && flags_due(T.scen[sid], C.it))
+        T.ctrl[sid].flags_it = C.it + 1;
       if (!(vmx <= kSpeedDiverged && vmy <= kSpeedDiverged))
         atomicOr(&T.ctrl[sid].diverged, 1);
     } else {
@@ -1673,29 +1703,34 @@ __global__ void __launch_bounds__(BS)
 }
 
 // ------------------------------------------------------------- merge/split
-// Fresh action flag of particle ip under its scenario's rule.
+// Fresh action flag of particle ip under its scenario's rule; from the
+// strain bits of the last particle update when it stored them for this
+// step (flags_due; the dt is then the fixed one the bits were taken with).
 template <typename R>
 __device__ __forceinline__ int fresh_flag(const Scen &sc, const Ctrl &C,
                                           const MsIn<R> &in, const ElfsCells &E,
-                                          const int *key, int ip) {
+                                          const int *key, const int *bits,
+                                          int ip) {
   if (sc.ms.mode == 1) {
     const double e = elfs_value(E, sc.coff, sc.P.nrows, sc.P.ncols, sc.P.hx,
                                 sc.P.hy, key[ip] - sc.coff, in.x[ip], in.y[ip]);
     return elfs_flag_one(sc.ms, e, in, ip);
   }
+  if (C.flags_it == C.it) return ms_flag_of_bits(sc.ms, bits[ip], in.hp[ip], in.level[ip]);
   return ms_flag_one(sc.ms, C.dt, in, ip);
 }
 
 template <typename R>
 __global__ void ens_flag_kernel(Tables T, MsIn<R> in, ElfsCells E,
-                                const int *__restrict__ key, int *act,
+                                const int *__restrict__ key,
+                                const int *__restrict__ bits, int *act,
                                 int *partner) {
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= T.poff[T.S]) return;
   const int sid = scen_of(T, ip);
   const Scen &sc = T.scen[sid];
   const Ctrl &C = T.ctrl[sid];
-  act[ip] = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, ip) : MS_KEEP;
+  act[ip] = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, bits, ip) : MS_KEEP;
   partner[ip] = -1;
 }
 
@@ -1847,7 +1882,8 @@ __device__ __forceinline__ bool caps_bind(const Scen &sc, const Ctrl &C,
 
 template <typename R>
 __global__ void ens_refresh_kernel(Tables T, MsIn<R> in, ElfsCells E,
-                                   const int *__restrict__ key, int *act,
+                                   const int *__restrict__ key,
+                                   const int *__restrict__ bits, int *act,
                                    int *partner, const int *cnt) {
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= T.poff[T.S]) return;
@@ -1855,7 +1891,7 @@ __global__ void ens_refresh_kernel(Tables T, MsIn<R> in, ElfsCells E,
   const Scen &sc = T.scen[sid];
   const Ctrl &C = T.ctrl[sid];
   if (!caps_bind(sc, C, cnt + 4 * sid)) return;
-  act[ip] = fresh_flag(sc, C, in, E, key, ip);
+  act[ip] = fresh_flag(sc, C, in, E, key, bits, ip);
   partner[ip] = -1;
 }
 
@@ -2132,7 +2168,8 @@ __device__ __forceinline__ int repack_src(const RepackMap &m, int i) {
 }
 
 template <typename R>
-__global__ void repack_particles_kernel(PPtr in, PPtr out, RepackMap m) {
+__global__ void repack_particles_kernel(PPtr in, PPtr out, RepackMap m,
+                                        const int *bits_in, int *bits_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.dst_off[m.nkeep]) return;
   const int j = repack_src(m, i);
@@ -2143,6 +2180,7 @@ __global__ void repack_particles_kernel(PPtr in, PPtr out, RepackMap m) {
   }
   out.label[i] = in.label[j];
   out.level[i] = in.level[j];
+  bits_out[i] = bits_in[j];  // merge/split strain bits a kept run may await
 }
 
 template <typename R>

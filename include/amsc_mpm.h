@@ -334,7 +334,11 @@ int amsc_sim_state(amsc_sim *s, int *it, double *t, double *dt);
 int amsc_sim_get_field(amsc_sim *s, const char *name, double *out, int n);
 /* Integer fields: label level p2g cptcls cstart. */
 int amsc_sim_get_field_int(amsc_sim *s, const char *name, int *out, int n);
-/* Overwrite a particle field from host memory (n == num_particles). */
+/* Overwrite a particle field from host memory (n == num_particles).
+ * AMSC_ERR_STALE for Ap, vxdx, vxdy, vydx and vydy after a step that is
+ * followed by a merge/split step of a fixed-dt run under the solvers' rule
+ * with aux output off: that step keeps them only for the particles whose
+ * strain tests passed. */
 int amsc_sim_set_field(amsc_sim *s, const char *name, const double *in,
                        int n);
 /* err_mass, err_mom (optim_par.cpp:395-428, FP64 device sums) and t of the

@@ -388,6 +388,47 @@ def test_repack_keeps_running_scenarios_bitwise():
     e.close()
 
 
+def test_repack_between_a_step_and_its_merge_split():
+    """A fixed-dt run whose next step merges and splits is moved by a repack
+    (to other particle indices) after the step that prepared its merge/split
+    decisions: it goes on exactly as if run alone, through that merge/split
+    and the next."""
+    from amsc_amd.simulation import Ensemble, Simulation
+    from amsc_amd import native
+    import ctypes as C
+    print("[ens] repack before merge/split", flush=True)
+    cases = mixed_cases()
+    b = cases[1]  # fixed dt, merge/split every 4 steps
+    cap = 8 * (len(cases[0][1]["x"]) + len(b[1]["x"]))
+    e = Ensemble([cases[0], b], "fp64", capacity=cap)
+    e.step(7)
+    h = C.c_void_p()
+    native.check(native.lib().amsc_ens_repack(
+        e._h, 1, np.array([1], np.int32), 0, None, None, *([None] * 11), cap,
+        C.byref(h)))
+    e._h = h
+    e.nscen, e.prms = 1, [b[0]]
+    e.nns = [(b[0]["nrows"] + 1) * (b[0]["ncols"] + 1)]
+    e.ncs = [b[0]["nrows"] * b[0]["ncols"]]
+    e.step(1)
+    s = Simulation(*b, capacity=cap)
+    s.step(8)
+    assert np.array_equal(e.ms_counts(0), s.ms_counts())
+    assert int(s.ms_counts().sum()) > 0
+    e.step(4)
+    e.set_aux(True)
+    e.step(1)
+    s.step(4)
+    s.set_aux(True)
+    s.step(1)
+    assert s.num_particles == e.num(0)
+    for f in FIELDS:
+        assert np.array_equal(bits(s.get(f)), bits(e.get(0, f))), f
+    assert np.array_equal(s.get_int("level"), e.get_int(0, "level"))
+    s.close()
+    e.close()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_served_requests_match_standalone_runs(graph):
     """Poisson arrivals served by the batched FP64 form (joins and leaves

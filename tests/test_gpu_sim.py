@@ -229,6 +229,50 @@ def test_merge_split_bitwise():
     assert splits > 50 and merges > 20, (splits, merges)
 
 
+def test_merge_split_without_outputs():
+    """With the optional outputs off, a step before a merge/split step of a
+    fixed-dt run passes its strain tests on as bits and keeps area, friction
+    and velocity gradients only where they passed: operation counts and the
+    particle state stay the oracle's bit for bit.  Until that merge/split,
+    Ap and the velocity gradients cannot be written (the step no longer
+    holds them all); the depth can, and counts at the merge/split."""
+    from amsc_amd.native import AmscError
+    from amsc_amd.simulation import Simulation
+    print("[sim] merge/split without outputs", flush=True)
+    d = small_slope_case()
+    d.update(MERGE_SPLIT_ON=1, ms_call_interval=4, ms_alpha=0.3, ms_beta=0.52,
+             ms_max_dv=2.0, ms_hp_min=0.3, ms_split_hp_min=0.2, NSTEPS=40)
+    prm = params_from_data(d)
+    part = particles_from_data(d)
+    o = oracle.OracleSim(prm, part, d["Z"])
+    g = Simulation(prm, part, d["Z"], capacity=8 * len(d["x"]))
+    core = ("x", "y", "vpx", "vpy", "hp", "Vp", "Mp")
+    splits = merges = 0
+    for k in range(1, 21):
+        assert o.step()
+        g.step(1)
+        if k % 4 == 3:
+            for f in ("Ap", "vpx_dx", "vpy_dy"):
+                with pytest.raises(AmscError) as ex:
+                    g.set(f, np.ones(g.num_particles))
+                assert ex.value.code == 5, f
+            if k == 11:
+                # a thinner layer: fewer merges (ms_hp_min) at step 12
+                hp = o.get("hp") * 0.5
+                o.set("hp", hp)
+                g.set("hp", hp)
+        if k % 4 == 0:
+            assert np.array_equal(g.ms_counts(), o.ms_counts()), f"step {k}"
+            splits += int(o.ms_counts()[0])
+            merges += int(o.ms_counts()[1])
+            assert g.num_particles == o.np
+            for f in core:
+                assert np.array_equal(bits(g.get(f)), bits(o.get(f))), (k, f)
+            assert np.array_equal(g.get_int("level"), o.get_int("level")), k
+    assert splits > 200 and merges > 200, (splits, merges)
+    g.close()
+
+
 def test_friction_force_recomputed_stored_and_rewritten():
     """P2G recomputes the friction force Ap*Fb from the stored particle
     state except after creation (Fb = 0), merge/split (mass-averaged Fb) and
