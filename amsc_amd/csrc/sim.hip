@@ -108,6 +108,7 @@ struct Sim {
   double *partials = nullptr;
   P2GOut *p2g_out[2] = {nullptr, nullptr};
   int *act = nullptr, *partner = nullptr, *wts = nullptr, *offs = nullptr;
+  int *ms_cand = nullptr;  // per cell: holds a merge candidate (kept zero between merge/split steps)
   int *ms_cnt = nullptr;
   // legacy ELFS rule only (ms_mode 1): exterior flags, hop distances and
   // two frontier queues, one int per cell each
@@ -457,12 +458,13 @@ int merge_split(Sim &s, bool capture) {
       return rc;
   if (n > 0)
     if (int rc = launch(s, "ms_flag", ens_flag_kernel<R>, gp, b256, T, in, E,
-                        key, (const int *)s.wts, s.act, s.partner))
+                        key, (const int *)s.wts, s.act, s.partner, s.ms_cnt,
+                        s.ms_cand))
       return rc;
   if (int rc = launch(s, "ms_pair", ens_pair_kernel<R>, dim3(grid_for(s.nc, 256)),
                       b256, T, (const int *)s.cell_off, s.nc, in,
                       (const int *)s.bin.cstart, (const int *)s.cptcls, s.act,
-                      s.partner, s.ms_cnt))
+                      s.partner, s.ms_cnt, s.ms_cand))
     return rc;
   if (n > 0)
     if (int rc = launch(s, "ms_refresh", ens_refresh_kernel<R>, gp, b256, T,
@@ -966,6 +968,8 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
     if (int rc = dalloc(*s, q, cap)) return rc;
   if (int rc = dalloc(*s, &s->offs, cap + 1)) return rc;
   if (int rc = dalloc(*s, &s->ms_cnt, 4 * S)) return rc;
+  if (int rc = dalloc(*s, &s->ms_cand, std::max(s->nc, 1))) return rc;
+  if (int rc = launch_fill_int(*s, s->ms_cand, std::max(s->nc, 1), 0)) return rc;
   for (int i = 0; i < S; ++i) s->any_elfs = s->any_elfs || prm[i].ms_mode == 1;
   if (s->any_elfs)
     for (int **q : {&s->elfs_ext, &s->elfs_hops, &s->elfs_qa, &s->elfs_qb})
@@ -1627,7 +1631,7 @@ long long bytes_estimate(int S, const amsc_params *prm, const int *np,
   for (int i = 0; i < S; ++i) elfs = elfs || prm[i].ms_mode == 1;
   // + upload staging: 3 FP64 values per initial particle
   return cap * per_p + npt * 24 + (cap / 64 + 1) * 4 + (cap / 9 + 2) * 4 +
-         nnt * per_n + nct * (elfs ? 24 : 8) +
+         nnt * per_n + nct * (elfs ? 28 : 12) +
          ntl * (6 * 8 + 4 * kTileCfgs) +
          S * (long long)(sizeof(Ctrl) + kTileCfgs * sizeof(Scen) + 40) +
          (1 << 16);  // allocations are rounded up to 256 bytes

@@ -1812,18 +1812,32 @@ __device__ __forceinline__ int fresh_flag(const Scen &sc, const Ctrl &C,
   return ms_flag_one(sc.ms, C.dt, in, ip);
 }
 
+// Fresh flags of every particle; splits counted per scenario (cnt[4*s]),
+// and the cells holding a merge candidate marked in cand (the pair kernel
+// visits only those and clears the mark).
 template <typename R>
 __global__ void ens_flag_kernel(Tables T, MsIn<R> in, ElfsCells E,
                                 const int *__restrict__ key,
                                 const int *__restrict__ bits, int *act,
-                                int *partner) {
+                                int *partner, int *cnt, int *cand) {
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= T.poff[T.S]) return;
   const int sid = scen_of(T, ip);
   const Scen &sc = T.scen[sid];
   const Ctrl &C = T.ctrl[sid];
-  act[ip] = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, bits, ip) : MS_KEEP;
+  const int f = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, bits, ip) : MS_KEEP;
+  act[ip] = f;
   partner[ip] = -1;
+  if (f == MS_CANDIDATE) cand[key[ip]] = 1;
+  // one atomic per wavefront when its particles share a scenario
+  const int s0 = __builtin_amdgcn_readfirstlane(sid);
+  const unsigned long long sp = __ballot(f == MS_SPLIT);
+  if (__ballot(sid != s0) == 0ull) {
+    if (sp && (int)__lane_id() == __ffsll((long long)__ballot(true)) - 1)
+      atomicAdd(&cnt[4 * s0], __popcll(sp));
+  } else if (f == MS_SPLIT) {
+    atomicAdd(&cnt[4 * sid], 1);
+  }
 }
 
 // The two cell passes of the legacy ELFS rule (merge_split_ops.h:86-262) for
@@ -1929,18 +1943,20 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-// One thread per cell: greedy pairing and the min-particles undo; counts
-// per scenario in cnt[4*s + 0..2] (splits, pairs formed, net merges).
+// One thread per cell holding a merge candidate (marked by the flag
+// kernel; the mark is cleared here): greedy pairing and the min-particles
+// undo (which only a pair can make necessary); counts per scenario in
+// cnt[4*s + 1..2] (pairs formed, net merges).
 template <typename R>
 __global__ void ens_pair_kernel(Tables T, const int *__restrict__ cell_off,
                                 int nc, MsIn<R> in,
                                 const int *__restrict__ cstart,
                                 const int *__restrict__ cptcls, int *act,
-                                int *partner, int *cnt) {
+                                int *partner, int *cnt, int *cand) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= nc) return;
+  if (cell >= nc || !cand[cell]) return;
+  cand[cell] = 0;
   const int b = cstart[cell], e = cstart[cell + 1];
-  if (b == e) return;
   int lo = 0, hi = T.S - 1;  // last scenario with cell_off <= cell
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1948,15 +1964,12 @@ __global__ void ens_pair_kernel(Tables T, const int *__restrict__ cell_off,
     else hi = mid - 1;
   }
   const Scen &sc = T.scen[lo];
-  if (!ms_due(sc, T.ctrl[lo])) return;
-  int nsplit = 0;
-  for (int i = b; i < e; ++i) nsplit += act[cptcls[i]] == MS_SPLIT;
   const int formed =
       pair_cell(b, e, cptcls, in, sc.ms.max_dv2, act, partner, nullptr, 0);
-  const int undone = undo_cell(b, e, cptcls, cell_min_ppc(sc.ms),
-                               split_weight(sc.ms), act, partner);
+  const int undone = formed ? undo_cell(b, e, cptcls, cell_min_ppc(sc.ms),
+                                        split_weight(sc.ms), act, partner)
+                            : 0;
   int *c = cnt + 4 * lo;
-  if (nsplit) atomicAdd(&c[0], nsplit);
   if (formed) atomicAdd(&c[1], formed);
   if (formed - undone) atomicAdd(&c[2], formed - undone);
 }
