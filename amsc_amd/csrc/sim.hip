@@ -526,6 +526,9 @@ int merge_split(Sim &s, bool capture) {
   ea.offset = s.offs;
   ea.err = s.err;
   ea.ncopy = s.aux_on ? PF_COUNT : PF_CORE;
+  ea.key = s.key[s.kcur];
+  ea.slot = s.slot;
+  ea.count = s.bin.count;
   if (n > 0)
     if (int rc = s.any_elfs
                      ? launch(s, "ms_execute", ms_execute_kernel<R, true>, gp, b256, ea)
@@ -542,7 +545,11 @@ int merge_split(Sim &s, bool capture) {
     s.poff_h.assign(h, h + s.S + 1);
     s.np = s.poff_h[s.S];
   }
-  return rebin(s, capture);
+  // the execute kernel took the new particles' keys and slots
+  Timer tm(s, "binning");
+  return launch_finish_binning(s.bin, capture ? s.cap : s.np, s.key[s.kcur],
+                               s.slot, s.cptcls, nullptr, s.st,
+                               capture ? s.poff + s.S : nullptr);
 }
 
 template <typename R, bool DESC>

@@ -2077,6 +2077,9 @@ struct ExecArgs {
   const int *act, *partner, *offset;
   const int *err;
   int ncopy;  // fields copied for kept particles (PF_CORE or PF_COUNT)
+  // the binning of the new particle set: cell keys, slots, cell counts
+  // (zero on entry)
+  int *key, *slot, *count;
 };
 
 template <typename R>
@@ -2097,7 +2100,7 @@ __device__ __forceinline__ void stv(const PPtr &p, int k, int i, double v) {
 // and the stored friction force Ap * Fb follow the new fields.
 template <typename R>
 __device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
-                                 double Lx, double Ly) {
+                                 double Lx, double Ly, double *ox, double *oy) {
   const PPtr &in = a.in, &out = a.out;
   if (act == MS_SPLIT) {
     const double px = ldv<R>(in, PF_X, i), py = ldv<R>(in, PF_Y, i);
@@ -2127,6 +2130,8 @@ __device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int oc = o + c;
+      ox[c] = cx[c];
+      oy[c] = cy[c];
       stv<R>(out, PF_X, oc, cx[c]);
       stv<R>(out, PF_Y, oc, cy[c]);
       stv<R>(out, PF_MOMX, oc, mx);
@@ -2149,6 +2154,8 @@ __device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
                  : (M1 * ldv<R>(in, k, i) + M2 * ldv<R>(in, k, j)) / Mt);
     stv<R>(out, PF_X, o, nx);
     stv<R>(out, PF_Y, o, ny);
+    ox[0] = nx;
+    oy[0] = ny;
     const double Mp = ldv<R>(out, PF_MP, o);
     stv<R>(out, PF_MOMX, o, Mp * ldv<R>(out, PF_VPX, o));
     stv<R>(out, PF_MOMY, o, Mp * ldv<R>(out, PF_VPY, o));
@@ -2160,30 +2167,33 @@ __device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
   }
 }
 
-// ELFS: some scenario of the batch uses the legacy ELFS rule (its code is
-// compiled only into that variant).
+// The new particles made from particle i (at o = offset[i]): a copy, the
+// two (four: ELFS rule) children of a split, the merge of a primary with
+// its partner, nothing for a secondary.  Returns how many, with their
+// positions in ox, oy.
 template <typename R, bool ELFS>
-__global__ void ms_execute_kernel(ExecArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (*a.err || i >= a.T.poff[a.T.S]) return;
-  const int act = a.act[i];
-  const int o = a.offset[i];
+__device__ __forceinline__ int execute_one(const ExecArgs &a, const Scen &sc,
+                                           int i, int o, int act, double *ox,
+                                           double *oy) {
   if (act == MS_KEEP) {
-    for (int k = 0; k < a.ncopy; ++k) {
+    ox[0] = fld<double>(a.in, PF_X)[i];
+    oy[0] = fld<double>(a.in, PF_Y)[i];
+    fld<double>(a.out, PF_X)[o] = ox[0];
+    fld<double>(a.out, PF_Y)[o] = oy[0];
+    for (int k = PF_Y + 1; k < a.ncopy; ++k) {
       if (pf_wide(k)) fld<double>(a.out, k)[o] = fld<double>(a.in, k)[i];
       else fld<R>(a.out, k)[o] = fld<R>(a.in, k)[i];
     }
     a.out.label[o] = a.in.label[i];
     a.out.level[o] = a.in.level[i];
-    return;
+    return 1;
   }
-  if (act != MS_SPLIT && act != MS_PRIMARY) return;
-  const Scen &sc = a.T.scen[scen_of(a.T, i)];
+  if (act != MS_SPLIT && act != MS_PRIMARY) return 0;
   const double Lx = sc.P.ncols * sc.P.hx, Ly = sc.P.nrows * sc.P.hy, eps = 1e-10;
   const PPtr &in = a.in, &out = a.out;
   if (ELFS && sc.ms.mode == 1) {
-    elfs_execute_one<R>(a, i, o, act, Lx, Ly);
-    return;
+    elfs_execute_one<R>(a, i, o, act, Lx, Ly, ox, oy);
+    return act == MS_SPLIT ? 4 : 1;
   }
   if (act == MS_SPLIT) {
     const int o1 = o, o2 = o + 1;
@@ -2213,6 +2223,10 @@ __global__ void ms_execute_kernel(ExecArgs a) {
     stv<R>(out, PF_Y, o1, y1);
     stv<R>(out, PF_X, o2, x2);
     stv<R>(out, PF_Y, o2, y2);
+    ox[0] = x1;
+    oy[0] = y1;
+    ox[1] = x2;
+    oy[1] = y2;
     const double Mc = ldv<R>(out, PF_MP, o1);
     const double mx = Mc * ldv<R>(out, PF_VPX, o1);
     const double my = Mc * ldv<R>(out, PF_VPY, o1);
@@ -2227,6 +2241,7 @@ __global__ void ms_execute_kernel(ExecArgs a) {
     stv<R>(out, PF_FRY, o2, Ap1 * Fby);
     out.label[o1] = out.label[o2] = -1;
     out.level[o1] = out.level[o2] = in.level[i] - 1;
+    return 2;
   } else {
     const int j = a.partner[i];
     const double M1 = ldv<R>(in, PF_MP, i), M2 = ldv<R>(in, PF_MP, j);
@@ -2241,6 +2256,8 @@ __global__ void ms_execute_kernel(ExecArgs a) {
     stv<R>(out, PF_HP, o, ldv<R>(out, PF_VP, o) / ldv<R>(out, PF_AP, o));
     stv<R>(out, PF_X, o, nx);
     stv<R>(out, PF_Y, o, ny);
+    ox[0] = nx;
+    oy[0] = ny;
     const double Mp = ldv<R>(out, PF_MP, o);
     stv<R>(out, PF_MOMX, o, Mp * ldv<R>(out, PF_VPX, o));
     stv<R>(out, PF_MOMY, o, Mp * ldv<R>(out, PF_VPY, o));
@@ -2249,7 +2266,45 @@ __global__ void ms_execute_kernel(ExecArgs a) {
     stv<R>(out, PF_FRY, o, Ap * ldv<R>(out, PF_FBY, o));
     out.label[o] = -1;
     out.level[o] = in.level[i] + 1;
+    return 1;
   }
+}
+
+// ELFS: some scenario of the batch uses the legacy ELFS rule (its code is
+// compiled only into that variant).  One thread per particle; the new
+// particles are binned as they are written (cell key and slot, the
+// re-binning's first pass).
+template <typename R, bool ELFS>
+__global__ void ms_execute_kernel(ExecArgs a) {
+  if (*a.err) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < a.T.poff[a.T.S];
+  int nout = 0, o = 0, sid = 0;
+  double ox[4], oy[4];
+  if (live) {
+    sid = scen_of(a.T, i);
+    o = a.offset[i];
+    nout = execute_one<R, ELFS>(a, a.T.scen[sid], i, o, a.act[i], ox, oy);
+  }
+  const Scen &sc = a.T.scen[sid];
+  int m = 0;
+#pragma unroll
+  for (int c = 0; c < (ELFS ? 4 : 2); ++c) {
+    const bool v = c < nout;
+    int k = 0;
+    if (v)
+      k = sc.coff + cell_of(ox[c], oy[c], sc.P.nrows, sc.P.ncols, sc.P.hx,
+                            sc.P.hy, sc.rc);
+    const int sl = wave_slot(a.count, k, v);
+    if (v) {
+      a.key[o + c] = k;
+      a.slot[o + c] = sl;
+      m = sl + 1 > m ? sl + 1 : m;
+    }
+  }
+  // the largest cell count, raised in the control block past the value
+  // stored there
+  if (m > a.T.ctrl[sid].max_ppc) atomicMax(&a.T.ctrl[sid].max_ppc, m);
 }
 
 // ------------------------------------------------------------- repack
