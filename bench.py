@@ -550,6 +550,7 @@ def main():
                  "step_roofline_frac": round(mb / (rm["ms_per_step"] * 1e-3) / 1e9
                                              / HBM_PEAK_GBS, 4),
                  "tiling": {"p2g_variant": rm["variant"], "particle_block": rm["block"]},
+                 "autotune_ms": rm["tune_ms"],
                  "kernels_avg_ms": rm["kernels"]}
         from amsc_amd.data import params_from_data
         em = field_errors(grab(simm, 0), state64, params_from_data(d))

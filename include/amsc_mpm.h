@@ -379,7 +379,9 @@ int amsc_sim_set_graph(amsc_sim *s, int on);
 int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block);
 /* Autotuner: after one untimed step, time every P2G variant, then every
  * workgroup size, over steps_per_variant live steps each (one step of each
- * candidate per round) and keep the fastest.  The steps are real steps of
+ * candidate per round) and keep the fastest (a swept variant only if it is
+ * 5 % faster than the best tiled one: the swept kernel slows down as the
+ * particles of a flow spread unevenly over its cells).  The steps are real steps of
  * the run; *steps_used (may be NULL) receives their number,
  * 1 + steps_per_variant * (AMSC_P2G_VARIANTS + 2); variant_ms (may be
  * NULL, AMSC_P2G_VARIANTS + 2 entries) the average P2G time of each
