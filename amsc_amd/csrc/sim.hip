@@ -632,7 +632,7 @@ P2GArgs p2g_args(Sim &s, int aux, bool partials) {
 template <typename R, int AUX, int BS>
 int launch_particle_aux(Sim &s, const PartArgs &a, int n) {
   if (int rc = launch(s, "particle", particle_kernel<R, AUX, BS, false>,
-                      dim3(grid_for(n, kPartIlp<R> * BS)), dim3(BS), a))
+                      dim3(grid_for(n, kPartIlp * BS)), dim3(BS), a))
     return rc;
   if (s.S == 1) return AMSC_OK;  // no scenario boundaries
   return launch(s, "particle_edge", particle_kernel<R, AUX, BS, true>,

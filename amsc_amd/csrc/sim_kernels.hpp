@@ -1632,13 +1632,13 @@ __device__ __forceinline__ void particle_body(const PartArgs &a, const Scen &sc,
   particle_update<R, AUX>(a, sc, dt, it, ip, in, newkey, vmx, vmy, hmx);
 }
 
-// Particles per thread of the particle kernel: a thread loads the inputs
+// Particles per thread of the particle kernel (a thread loads the inputs
 // of all of them before it updates the first, so that their memory round
-// trips overlap.  Two in the mixed form; in FP64 the second particle's
-// registers cost more waves than the overlap gains (measured on the 4M
-// workload: 0.4245 ms per step with one, 0.4312 with two; mixed 0.3760 and
-// 0.3709).
-template <typename R> constexpr int kPartIlp = sizeof(R) == 8 ? 1 : 2;
+// trips overlap).  One: two measured slower in both precisions on the 4M
+// workload, 0.3910 against 0.3760 ms per FP64 step and 0.3202 against
+// 0.3106 mixed (the second particle's registers cost more waves than the
+// overlap gains).
+constexpr int kPartIlp = 1;
 
 // kPartIlp particles per thread (thread t of a workgroup of BS takes
 // particles t, t + BS, ... of the workgroup's kPartIlp * BS).  Particles of
@@ -1654,7 +1654,7 @@ template <typename R, int AUX, int BS, bool EDGE>
 __global__ void __launch_bounds__(BS)
     __attribute__((amdgpu_waves_per_eu(EDGE ? 1 : AUX == AUX_FULL ? 4 : AUX == AUX_MS ? 5 : 6)))
     particle_kernel(PartArgs a) {
-  constexpr int ILP = kPartIlp<R>, PB = ILP * BS;
+  constexpr int ILP = kPartIlp, PB = ILP * BS;
   const Tables &T = a.T;
   const int np = T.poff[T.S];
   int blk = blockIdx.x;
