@@ -430,8 +430,11 @@ int rebin(Sim &s, bool capture) {
                                nullptr, s.st, capture ? s.poff + s.S : nullptr);
 }
 
+// *all_ended (eager launches only): every run had ended before this step;
+// nothing was changed.
 template <typename R>
-int merge_split(Sim &s, bool capture) {
+int merge_split(Sim &s, bool capture, bool *all_ended) {
+  *all_ended = false;
   const Tables T = tables(s);
   PPtr &p = s.buf[s.cur];
   MsIn<R> in;
@@ -493,16 +496,20 @@ int merge_split(Sim &s, bool capture) {
                       s.poff_next, s.err))
     return rc;
   if (!capture) {
-    // one read-back: operation counts, overflow flag, new offsets
+    // one read-back: operation counts, overflow flag, new offsets, and the
+    // control blocks' headers (whether any run is still going)
     int *h = s.host_ints;
-    AMSC_HIP(hipMemcpyAsync(h, s.ms_cnt, sizeof(int) * 4 * s.S,
+    AMSC_HIP(hipMemcpyAsync(h, s.ms_cnt, sizeof(int) * (5 * s.S + 2),
                             hipMemcpyDeviceToHost, s.st));
-    AMSC_HIP(hipMemcpyAsync(h + 4 * s.S, s.err, sizeof(int),
-                            hipMemcpyDeviceToHost, s.st));
-    AMSC_HIP(hipMemcpyAsync(h + 4 * s.S + 1, s.poff_next,
-                            sizeof(int) * (s.S + 1), hipMemcpyDeviceToHost,
-                            s.st));
+    AMSC_HIP(hipMemcpy2DAsync(s.ctrl_host, sizeof(Ctrl), s.ctrl, sizeof(Ctrl),
+                              kCtrlHead, s.S, hipMemcpyDeviceToHost, s.st));
     AMSC_HIP(hipStreamSynchronize(s.st));
+    bool active = false;
+    for (int i = 0; i < s.S; ++i) active = active || s.ctrl_host[i].active;
+    if (!active) {
+      *all_ended = true;
+      return AMSC_OK;
+    }
     bool any = false;
     for (int i = 0; i < s.S; ++i) {
       s.ms_counts[2 * i] = h[4 * i];
@@ -656,17 +663,13 @@ int step_body(Sim &s, bool ms_step, int aux, bool capture, bool *skipped) {
                       (const Scen *)s.scen[s.tile_cfg], s.ctrl))
     return rc;
   if (ms_step) {
-    if (!capture) {
-      if (int rc = read_ctrl(s, false)) return rc;
-      bool any = false;
-      for (int i = 0; i < s.S; ++i) any = any || s.ctrl_host[i].active;
-      if (!any) {
-        s.finished = true;
-        *skipped = true;
-        return AMSC_OK;
-      }
+    bool all_ended = false;
+    if (int rc = merge_split<R>(s, capture, &all_ended)) return rc;
+    if (all_ended) {
+      s.finished = true;
+      *skipped = true;
+      return AMSC_OK;
     }
-    if (int rc = merge_split<R>(s, capture)) return rc;
   }
   const Tables T = tables(s);
   PPtr &p = s.buf[s.cur];
@@ -740,9 +743,7 @@ int launch_graph(Sim &s, bool ms_step, int aux) {
   s.kcur = it->second.kcur_after;
   if (ms_step) {
     // particle counts and operation counts of the step, and overflow
-    AMSC_HIP(hipMemcpyAsync(s.host_ints, s.ms_cnt, sizeof(int) * 4 * s.S,
-                            hipMemcpyDeviceToHost, s.st));
-    AMSC_HIP(hipMemcpyAsync(s.host_ints + 4 * s.S, s.err, sizeof(int),
+    AMSC_HIP(hipMemcpyAsync(s.host_ints, s.ms_cnt, sizeof(int) * (4 * s.S + 1),
                             hipMemcpyDeviceToHost, s.st));
     AMSC_HIP(hipStreamSynchronize(s.st));
     for (int i = 0; i < s.S; ++i) {
@@ -974,7 +975,11 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   for (int **q : ints)
     if (int rc = dalloc(*s, q, cap)) return rc;
   if (int rc = dalloc(*s, &s->offs, cap + 1)) return rc;
-  if (int rc = dalloc(*s, &s->ms_cnt, 4 * S)) return rc;
+  // operation counts, overflow flag and new offsets of a merge/split step
+  // in one block, read back with one copy
+  if (int rc = dalloc(*s, &s->ms_cnt, 5 * S + 2)) return rc;
+  s->err = s->ms_cnt + 4 * S;
+  s->poff_next = s->err + 1;
   if (int rc = dalloc(*s, &s->ms_cand, std::max(s->nc, 1))) return rc;
   if (int rc = launch_fill_int(*s, s->ms_cand, std::max(s->nc, 1), 0)) return rc;
   for (int i = 0; i < S; ++i) s->any_elfs = s->any_elfs || prm[i].ms_mode == 1;
@@ -982,10 +987,8 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
     for (int **q : {&s->elfs_ext, &s->elfs_hops, &s->elfs_qa, &s->elfs_qb})
       if (int rc = dalloc(*s, q, std::max(s->nc, 1))) return rc;
   if (int rc = dalloc(*s, &s->poff, S + 1)) return rc;
-  if (int rc = dalloc(*s, &s->poff_next, S + 1)) return rc;
   if (int rc = dalloc(*s, &s->wscen, cap / 64 + 1)) return rc;
   if (int rc = dalloc(*s, &s->cell_off, S + 1)) return rc;
-  if (int rc = dalloc(*s, &s->err, 1)) return rc;
   for (int k = 0; k < NF_COUNT; ++k) {
     char *q = nullptr;
     if (int rc = dalloc(*s, &q, nf_bytes(*s, k) * (size_t)s->nn)) return rc;
