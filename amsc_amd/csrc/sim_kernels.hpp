@@ -2178,21 +2178,39 @@ __device__ void elfs_execute_one(const ExecArgs &a, int i, int o, int act,
 // two (four: ELFS rule) children of a split, the merge of a primary with
 // its partner, nothing for a secondary.  Returns how many, with their
 // positions in ox, oy.
+// A kept particle's fields [0, NC) copied from i to o: every load issued
+// before the first store, so that the round trips overlap.  *px, *py: its
+// position.
+template <typename R, int NC>
+__device__ __forceinline__ void copy_fields(const PPtr &in, const PPtr &out,
+                                            int i, int o, double *px,
+                                            double *py) {
+  double vw[NC];
+  R vn[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    if (pf_wide(k)) vw[k] = fld<double>(in, k)[i];
+    else vn[k] = fld<R>(in, k)[i];
+  }
+  const int lb = in.label[i], lv = in.level[i];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    if (pf_wide(k)) fld<double>(out, k)[o] = vw[k];
+    else fld<R>(out, k)[o] = vn[k];
+  }
+  out.label[o] = lb;
+  out.level[o] = lv;
+  *px = vw[PF_X];
+  *py = vw[PF_Y];
+}
+
 template <typename R, bool ELFS>
 __device__ __forceinline__ int execute_one(const ExecArgs &a, const Scen &sc,
                                            int i, int o, int act, double *ox,
                                            double *oy) {
   if (act == MS_KEEP) {
-    ox[0] = fld<double>(a.in, PF_X)[i];
-    oy[0] = fld<double>(a.in, PF_Y)[i];
-    fld<double>(a.out, PF_X)[o] = ox[0];
-    fld<double>(a.out, PF_Y)[o] = oy[0];
-    for (int k = PF_Y + 1; k < a.ncopy; ++k) {
-      if (pf_wide(k)) fld<double>(a.out, k)[o] = fld<double>(a.in, k)[i];
-      else fld<R>(a.out, k)[o] = fld<R>(a.in, k)[i];
-    }
-    a.out.label[o] = a.in.label[i];
-    a.out.level[o] = a.in.level[i];
+    if (a.ncopy == PF_COUNT) copy_fields<R, PF_COUNT>(a.in, a.out, i, o, ox, oy);
+    else copy_fields<R, PF_CORE>(a.in, a.out, i, o, ox, oy);
     return 1;
   }
   if (act != MS_SPLIT && act != MS_PRIMARY) return 0;
