@@ -1830,16 +1830,26 @@ __global__ void ens_flag_kernel(Tables T, MsIn<R> in, ElfsCells E,
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= T.poff[T.S]) return;
   const int sid = scen_of(T, ip);
-  const Scen &sc = T.scen[sid];
-  const Ctrl &C = T.ctrl[sid];
-  const int f = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, bits, ip) : MS_KEEP;
+  // a wavefront whose particles share a scenario reads its parameters with
+  // scalar loads
+  const int s0 = __builtin_amdgcn_readfirstlane(sid);
+  const bool uniform = __ballot(sid != s0) == 0ull;
+  int f;
+  if (uniform) {
+    const Scen &sc = T.scen[s0];
+    const Ctrl &C = T.ctrl[s0];
+    f = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, bits, ip) : MS_KEEP;
+  } else {
+    const Scen &sc = T.scen[sid];
+    const Ctrl &C = T.ctrl[sid];
+    f = ms_due(sc, C) ? fresh_flag(sc, C, in, E, key, bits, ip) : MS_KEEP;
+  }
   act[ip] = f;
   partner[ip] = -1;
   if (f == MS_CANDIDATE) cand[key[ip]] = 1;
   // one atomic per wavefront when its particles share a scenario
-  const int s0 = __builtin_amdgcn_readfirstlane(sid);
   const unsigned long long sp = __ballot(f == MS_SPLIT);
-  if (__ballot(sid != s0) == 0ull) {
+  if (uniform) {
     if (sp && (int)__lane_id() == __ffsll((long long)__ballot(true)) - 1)
       atomicAdd(&cnt[4 * s0], __popcll(sp));
   } else if (f == MS_SPLIT) {
@@ -2000,6 +2010,11 @@ __global__ void ens_refresh_kernel(Tables T, MsIn<R> in, ElfsCells E,
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= T.poff[T.S]) return;
   const int sid = scen_of(T, ip);
+  // most wavefronts share one scenario whose caps do not bind: they leave
+  // after scalar reads
+  const int s0 = __builtin_amdgcn_readfirstlane(sid);
+  if (__ballot(sid != s0) == 0ull && !caps_bind(T.scen[s0], T.ctrl[s0], cnt + 4 * s0))
+    return;
   const Scen &sc = T.scen[sid];
   const Ctrl &C = T.ctrl[sid];
   if (!caps_bind(sc, C, cnt + 4 * sid)) return;
