@@ -555,6 +555,9 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
   // stored friction force (first step, after merge/split or a field
   // write) or Ap * Fb recomputed from the stored vpx, vpy, hp, Vp
   const bool fric_read = ctrl->fric_read || ms_due(sc, *ctrl);
+  // read before the kernel's first store: a load of it after one is a
+  // vector load that the node update waits for, column after column
+  const double step_dt = ctrl->dt;
   const int nrows = sc.P.nrows, ncols = sc.P.ncols, bc = sc.P.bc;
   const double hx = sc.P.hx, hy = sc.P.hy, grav = sc.P.g, rho = sc.P.rho;
   const int coff = sc.coff, noff = sc.noff, toff = sc.toff, ntx = sc.ntx;
@@ -793,7 +796,7 @@ __global__ void __launch_bounds__(TC * TR) p2g_nodal_kernel(P2GArgs a) {
 
   if (node_ok && (L > 0 || a.aux >= AUX_FULL))
     p2g_node_update<R>(a, noff + r + c * (nrows + 1), c, r, nrows, ncols, bc,
-                       ctrl->dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry, Fix, Fiy);
+                       step_dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry, Fix, Fiy);
 
   if (a.partials) {
     // conservation sums (optim_par.cpp:391-428), FP64, fixed order: tile
@@ -870,6 +873,9 @@ __device__ __forceinline__ void p2g_sweep_body(P2GArgs a) {
   Ctrl *ctrl = &a.T.ctrl[sid];
   if (!ctrl->active || (a.dense && a.dense[sid])) return;
   const bool fric_read = ctrl->fric_read || ms_due(sc, *ctrl);
+  // read before the kernel's first store: a load of it after one is a
+  // vector load that the node update waits for, column after column
+  const double step_dt = ctrl->dt;
   const int nrows = sc.P.nrows, ncols = sc.P.ncols, bc = sc.P.bc;
   const double hx = sc.P.hx, hy = sc.P.hy, grav = sc.P.g, rho = sc.P.rho;
   const int coff = sc.coff, noff = sc.noff, toff = sc.toff, ntx = sc.ntx;
@@ -1187,7 +1193,7 @@ __device__ __forceinline__ void p2g_sweep_body(P2GArgs a) {
     if (!right && node_ok) {
       if (any || a.aux >= AUX_FULL)
         p2g_node_update<R>(a, noff + r + c * (nrows + 1), c, r, nrows, ncols, bc,
-                           ctrl->dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry, Fix, Fiy);
+                           step_dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry, Fix, Fiy);
       nm += Mv;
       nmx += mvx;
       nmy += mvy;
@@ -1321,6 +1327,7 @@ __global__ void __launch_bounds__((TC * TR + 63) / 64 * 64)
   const Scen &sc = a.T.scen[sid];
   const Ctrl *ctrl = &a.T.ctrl[sid];
   if (!ctrl->active) return;
+  const double step_dt = ctrl->dt;  // read before the first store
   const int nrows = sc.P.nrows, ncols = sc.P.ncols;
   const RecipT<A> rcr = recip_of<A>(sc);
   const int lt = tile - sc.toff;
@@ -1390,7 +1397,7 @@ __global__ void __launch_bounds__((TC * TR + 63) / 64 * 64)
       }
     }
     p2g_node_update<R>(a, sc.noff + r + c * (nrows + 1), c, r, nrows, ncols,
-                       sc.P.bc, ctrl->dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry,
+                       sc.P.bc, step_dt, Mv, mvx, mvy, FPx, FPy, Frx, Fry,
                        Fix, Fiy);
   }
   if (a.partials) {
@@ -1505,6 +1512,16 @@ __device__ __forceinline__ void particle_update(const PartArgs &a, const Scen &s
   const PPtr &p = a.p;
   const int nrows = P.nrows;
   const double hx = P.hx, hy = P.hy;
+  // the merge/split parameters read before the first store (after one they
+  // are vector loads, each waited for in the middle of the update)
+  const bool bits_due = AUX == AUX_MS && flags_due(sc, it);
+  const int coff = sc.coff;
+  MsParams msp{};
+  double dt_fixed = 0.0;
+  if (bits_due) {
+    msp = sc.ms;
+    dt_fixed = P.dt_fixed;
+  }
   const int cell = in.cell, c = fdiv(cell, sc.fd), r = cell - c * nrows;
   const A *gx = in.gx, *gy = in.gy;
   const double x = in.x, y = in.y, hp0 = in.hp0, H = in.H;
@@ -1614,9 +1631,9 @@ __device__ __forceinline__ void particle_update(const PartArgs &a, const Scen &s
     fld<R>(p, PF_FRY)[ip] = Ap * Fby;
   }
   bool ms_fields = AUX >= AUX_MS;
-  if (AUX == AUX_MS && flags_due(sc, it)) {
+  if (bits_due) {
     // the strain tests from the values as stored
-    const int bits = ms_strain_bits(sc.ms, P.dt_fixed, (double)(R)Ap,
+    const int bits = ms_strain_bits(msp, dt_fixed, (double)(R)Ap,
                                     (double)(R)vxdx, (double)(R)vxdy,
                                     (double)(R)vydx, (double)(R)vydy);
     a.ms_bits[ip] = bits;
@@ -1652,7 +1669,7 @@ __device__ __forceinline__ void particle_update(const PartArgs &a, const Scen &s
     fld<R>(p, PF_MOMX)[ip] = vpx * Mp;
     fld<R>(p, PF_MOMY)[ip] = vpy * Mp;
   }
-  newkey = sc.coff + cell_of(xn, yn, P.nrows, P.ncols, hx, hy, rc);
+  newkey = coff + cell_of(xn, yn, nrows, P.ncols, hx, hy, rc);
   vmx = fabs((double)vpx);
   vmy = fabs((double)vpy);
   hmx = h > 0.0 ? h : 0.0;
@@ -1754,7 +1771,12 @@ __global__ void __launch_bounds__(BS)
   // scenario's parameters are read with scalar loads
   const int sid = __builtin_amdgcn_readfirstlane(sb);
   const Ctrl &C = T.ctrl[sid];
+  // the control block's fields read before the first store (after one they
+  // would be read again with vector loads, each waited for)
   const bool act = C.active != 0;
+  const double cdt = C.dt;
+  const int cit = C.it, cmax = C.max_ppc;
+  const bool fdue = AUX == AUX_MS && flags_due(T.scen[sid], cit);
   int newkey[ILP];
   double vmx = 0.0, vmy = 0.0, hmx = 0.0;
   if (act) {
@@ -1771,7 +1793,7 @@ __global__ void __launch_bounds__(BS)
       newkey[j] = 0;
       if (ip < np) {
         double vx, vy, hm;
-        particle_update<R, AUX>(a, sc, C.dt, C.it, ip, in[j], newkey[j], vx, vy, hm);
+        particle_update<R, AUX>(a, sc, cdt, cit, ip, in[j], newkey[j], vx, vy, hm);
         if (!(vx <= kSpeedDiverged && vy <= kSpeedDiverged))
           atomicOr(&T.ctrl[sid].diverged, 1);
         vmx = fmax(vmx, vx);
@@ -1779,8 +1801,7 @@ __global__ void __launch_bounds__(BS)
         hmx = fmax(hmx, hm);
       }
     }
-    if (AUX == AUX_MS && threadIdx.x == 0 && flags_due(sc, C.it))
-      T.ctrl[sid].flags_it = C.it + 1;
+    if (fdue && threadIdx.x == 0) T.ctrl[sid].flags_it = cit + 1;
   } else {
 #pragma unroll
     for (int j = 0; j < ILP; ++j) {
@@ -1807,8 +1828,7 @@ __global__ void __launch_bounds__(BS)
     const int o = __shfl_xor(m, d);
     m = o > m ? o : m;
   }
-  if ((threadIdx.x & 63) == 0 && m > T.ctrl[sid].max_ppc)
-    atomicMax(&T.ctrl[sid].max_ppc, m);
+  if ((threadIdx.x & 63) == 0 && m > cmax) atomicMax(&T.ctrl[sid].max_ppc, m);
   if (!act) return;
   __shared__ double s_cfl[3][BS / 64];
 #pragma unroll
@@ -1830,7 +1850,7 @@ __global__ void __launch_bounds__(BS)
       vmy = fmax(vmy, s_cfl[1][k]);
       hmx = fmax(hmx, s_cfl[2][k]);
     }
-    unsigned long long *c = T.ctrl[sb].cfl[blk & 63][C.it & 1];
+    unsigned long long *c = T.ctrl[sb].cfl[blk & 63][cit & 1];
     atomicMax(&c[0], (unsigned long long)__double_as_longlong(vmx));
     atomicMax(&c[1], (unsigned long long)__double_as_longlong(vmy));
     atomicMax(&c[2], (unsigned long long)__double_as_longlong(hmx));
