@@ -123,7 +123,7 @@ struct Sim {
   std::vector<char> ended;     // run known to have ended, per scenario
   // default tiling: the fastest on every workload measured (8x32-node P2G
   // tiles in linear order, 512-thread particle workgroups)
-  int tile_cfg = 2, tile_order = 1, part_bs = 512;
+  int tile_cfg = 2, tile_order = 0, part_bs = 256;
   // scenarios whose P2G runs on the dense-cell kernels (decided on the host
   // from the control blocks' max_ppc, once a cell has held more than
   // kDensePPC particles), their flags on the device, and the staged values
@@ -1557,14 +1557,17 @@ int autotune(Sim &s, int steps, int *cfg_out, int *bs_out, int *used,
         s.tile_order = v / kTileCfgs;
       }, pv))
     return rc;
-  // The swept shape (configuration 1) is taken only if it beats the best
-  // tiled one by kSweepMargin: its columns of more than 128 positions go
-  // through an unpipelined path, and their share grows as a flow spreads
-  // and cells fill unevenly (on the 4M workload: none at step 60, 12 % of
-  // columns at step 120, 22 % at step 220), while the tiled kernels keep
-  // their speed.  Measured at a run's start, the two are often within a
-  // few per cent.
-  constexpr double kSweepMargin = 0.05;
+  // In the mixed form the swept shape (configuration 1) is taken only if it
+  // beats the best tiled one by kSweepMargin: its columns of more than 128
+  // positions go through an unpipelined path, and their share grows as a
+  // flow spreads and cells fill unevenly (on the 4M workload: none at step
+  // 60, 12 % of columns at step 120, 22 % at step 220), while the tiled
+  // kernels keep their speed (over 200 steps: 0.3167 against 0.3007 ms per
+  // step).  Measured at a run's start, the two are often within a few per
+  // cent.  The FP64 swept kernel loads the next 64 positions of such
+  // columns ahead and keeps its lead over the run (0.3619 against 0.3666
+  // ms per step), so it needs no margin.
+  const double kSweepMargin = s.mixed ? 0.05 : 0.0;
   int bv = v0, bt = -1;
   double best = 1e300, best_t = 1e300;
   for (int v = 0; v < kP2GVariants; ++v) {
@@ -1579,6 +1582,18 @@ int autotune(Sim &s, int steps, int *cfg_out, int *bs_out, int *used,
     }
   }
   if (bv % kTileCfgs == 1 && bt >= 0 && best > (1.0 - kSweepMargin) * best_t) bv = bt;
+  // Likewise the linear tile order (1) only if it beats the descending one
+  // (0) of the same shape by kOrderMargin.  The descending order stages
+  // first the particles that the previous update wrote last, the likeliest
+  // still in the memory-side cache; at a run's start the two time within a
+  // few per cent either way, over 200 steps of the 4M workload the linear
+  // order was the slower in every pair (FP64 swept 0.3686 against 0.3619
+  // ms per step, mixed 8x32 tiles 0.3051 against 0.3007).
+  constexpr double kOrderMargin = 0.03;
+  if (bv / kTileCfgs == 1) {
+    const int b0 = bv % kTileCfgs;
+    if (pv[b0] > 0.0 && pv[bv] > (1.0 - kOrderMargin) * pv[b0]) bv = b0;
+  }
   s.tile_cfg = bv % kTileCfgs;
   s.tile_order = bv / kTileCfgs;
   const int bs0 = s.part_bs;

@@ -374,14 +374,17 @@ int amsc_sim_set_graph(amsc_sim *s, int on);
  * swept by one wavefront column by column, 2: 8x32), tile order v / 3 (0:
  * column-major from the last tile column down, 1: linear); particle-kernel
  * workgroup size 256 or 512.  Results do not depend on either.  Default:
- * variant 5 (8x32, linear) and 512. */
+ * variant 2 (8x32, column-major from the last tile column down) and 256. */
 #define AMSC_P2G_VARIANTS 6
 int amsc_sim_set_tiling(amsc_sim *s, int p2g_shape, int particle_block);
 /* Autotuner: after one untimed step, time every P2G variant, then every
  * workgroup size, over steps_per_variant live steps each (one step of each
- * candidate per round) and keep the fastest (a swept variant only if it is
- * 5 % faster than the best tiled one: the swept kernel slows down as the
- * particles of a flow spread unevenly over its cells).  The steps are real steps of
+ * candidate per round) and keep the fastest (in mixed precision a swept
+ * variant only if it is 5 % faster than the best tiled one: that swept
+ * kernel slows down as the particles of a flow spread unevenly over its
+ * cells; the linear tile order
+ * only if it is 3 % faster than the other order of the same shape, which
+ * reads first the particles the previous step wrote last).  The steps are real steps of
  * the run; *steps_used (may be NULL) receives their number,
  * 1 + steps_per_variant * (AMSC_P2G_VARIANTS + 2); variant_ms (may be
  * NULL, AMSC_P2G_VARIANTS + 2 entries) the average P2G time of each
