@@ -35,6 +35,8 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
+#include <chrono>
 #include <string>
 #include <utility>
 #include <vector>
@@ -121,9 +123,18 @@ struct Sim {
   // particle update wrote it, or no step has run since it was set)
   std::vector<char> fric_cur;
   std::vector<char> ended;     // run known to have ended, per scenario
-  // default tiling: the fastest on every workload measured (8x32-node P2G
-  // tiles in linear order, 512-thread particle workgroups)
+  // default tiling (8x32-node P2G tiles from the last tile column down,
+  // 256-thread particle workgroups); the autotuner may pick another
   int tile_cfg = 2, tile_order = 0, part_bs = 256;
+  // merge/split read-back, eager steps: ms_spec = the execute and the
+  // re-binning are enqueued before the operation counts reach the host (set
+  // while the last merge/split step changed particles); the offsets kernel
+  // then writes the counts into ms_box (pinned, host-coherent), word 0 last
+  // (= ms_seq)
+  bool ms_spec = true;
+  int *ms_box = nullptr;
+  size_t ms_box_bytes = 0;
+  int ms_seq = 0;
   // scenarios whose P2G runs on the dense-cell kernels (decided on the host
   // from the control blocks' max_ppc, once a cell has held more than
   // kDensePPC particles), their flags on the device, and the staged values
@@ -342,6 +353,7 @@ void sim_free(Sim *s) {
   if (s->ctrl_host) pinned_give(s->ctrl_host, s->ctrl_host_bytes);
   if (s->host_ints) pinned_give(s->host_ints, s->host_ints_bytes);
   for (auto e : s->event_pool) (void)hipEventDestroy(e);
+  if (s->ms_box) (void)hipHostFree(s->ms_box);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
 }
@@ -491,24 +503,46 @@ int merge_split(Sim &s, bool capture, bool *all_ended) {
     if (int rc = exclusive_scan(s.wts, s.offs, n, s.ms_scan, s.st, nullptr))
       return rc;
   }
+  // Eager steps read back the operation counts, the overflow flag, the new
+  // offsets and the scenarios' active flags (whether any run is still
+  // going).  With ms_spec the execute, commit and re-binning are enqueued
+  // before the counts reach the host, and the offsets kernel writes them
+  // straight into host memory (word 0, the sequence number, last): the
+  // device goes on with the execute while the host takes them.  (Waiting
+  // for the stream left the device idle about 38 us per merge/split step
+  // on the 4M workload; a copy to the host inside the stream still held
+  // the next kernel back about 28 us after the copy had ended.)
+  // Speculation gives the same state: with no operation the execute copies
+  // every particle unchanged (so it is only used while the last merge/split
+  // step changed particles); on overflow the execute and the commit do
+  // nothing, the host keeps the current buffer and bins it again.
+  const bool spec = !capture && s.ms_spec;
+  const size_t nbox = 6 * (size_t)s.S + 3;  // sequence word + the block
+  if (spec && s.ms_box_bytes < sizeof(int) * nbox) {
+    if (s.ms_box) (void)hipHostFree(s.ms_box);
+    s.ms_box = nullptr;
+    s.ms_box_bytes = 0;
+    AMSC_HIP(hipHostMalloc((void **)&s.ms_box, sizeof(int) * nbox,
+                           hipHostMallocCoherent));
+    s.ms_box_bytes = sizeof(int) * nbox;
+    s.ms_box[0] = s.ms_seq;
+  }
+  if (spec) s.ms_seq = s.ms_seq == INT_MAX ? 1 : s.ms_seq + 1;
   if (int rc = launch(s, "ms_offsets", ms_offsets_kernel, dim3(1), b256,
                       (const int *)s.poff, s.S, (const int *)s.offs, s.cap,
-                      s.poff_next, s.err))
+                      s.poff_next, s.err, (const Ctrl *)s.ctrl,
+                      s.ms_cnt + 5 * s.S + 2, spec ? s.ms_box : nullptr,
+                      (const int *)s.ms_cnt, s.ms_seq))
     return rc;
-  if (!capture) {
-    // one read-back: operation counts, overflow flag, new offsets, and the
-    // control blocks' headers (whether any run is still going)
-    int *h = s.host_ints;
-    AMSC_HIP(hipMemcpyAsync(h, s.ms_cnt, sizeof(int) * (5 * s.S + 2),
-                            hipMemcpyDeviceToHost, s.st));
-    AMSC_HIP(hipMemcpy2DAsync(s.ctrl_host, sizeof(Ctrl), s.ctrl, sizeof(Ctrl),
-                              kCtrlHead, s.S, hipMemcpyDeviceToHost, s.st));
-    AMSC_HIP(hipStreamSynchronize(s.st));
+  const int *h = spec ? s.ms_box + 1 : s.host_ints;
+  // after the read-back: 0 go on, 1 nothing to do (all ended or no
+  // operation), AMSC_ERR_CAPACITY on overflow
+  auto take_counts = [&]() -> int {
     bool active = false;
-    for (int i = 0; i < s.S; ++i) active = active || s.ctrl_host[i].active;
+    for (int i = 0; i < s.S; ++i) active = active || h[5 * s.S + 2 + i];
     if (!active) {
       *all_ended = true;
-      return AMSC_OK;
+      return 1;
     }
     bool any = false;
     for (int i = 0; i < s.S; ++i) {
@@ -516,13 +550,23 @@ int merge_split(Sim &s, bool capture, bool *all_ended) {
       s.ms_counts[2 * i + 1] = h[4 * i + 2];
       any = any || h[4 * i] || h[4 * i + 2];
     }
-    if (!any) return AMSC_OK;  // no operation anywhere: nothing moves
+    s.ms_spec = any;
+    if (!any) return 1;  // no operation anywhere: nothing moves
     if (h[4 * s.S]) {
       (void)launch_fill_int(s, s.err, 1, 0);
       return fail(AMSC_ERR_CAPACITY,
                   "merge/split needs " + std::to_string(h[5 * s.S + 1]) +
                       " particles, capacity is " + std::to_string(s.cap));
     }
+    return 0;
+  };
+  if (!capture && !spec) {
+    AMSC_HIP(hipMemcpyAsync(s.host_ints, s.ms_cnt, sizeof(int) * (6 * s.S + 2),
+                            hipMemcpyDeviceToHost, s.st));
+    AMSC_HIP(hipStreamSynchronize(s.st));
+    const int t = take_counts();
+    if (t == 1) return AMSC_OK;
+    if (t) return t;
   }
   ExecArgs ea;
   ea.T = T;
@@ -546,10 +590,49 @@ int merge_split(Sim &s, bool capture, bool *all_ended) {
     return rc;
   if (int rc = update_wscen(s)) return rc;
   s.cur = 1 - s.cur;
+  if (spec) {
+    // the new particles binned before their count is known: launch extents
+    // from the capacity, the count read on the device
+    {
+      Timer tm(s, "binning");
+      if (int rc = launch_finish_binning(s.bin, s.cap, s.key[s.kcur], s.slot,
+                                         s.cptcls, nullptr, s.st, s.poff + s.S))
+        return rc;
+    }
+    // wait for the sequence word (bounded: past 10 s the stream is waited
+    // for, and the block copied the ordinary way if the word still lags)
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(&s.ms_box[0], __ATOMIC_ACQUIRE) != s.ms_seq) {
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10)) {
+        AMSC_HIP(hipStreamSynchronize(s.st));
+        if (__atomic_load_n(&s.ms_box[0], __ATOMIC_ACQUIRE) != s.ms_seq) {
+          AMSC_HIP(hipMemcpy(s.ms_box + 1, s.ms_cnt, sizeof(int) * (6 * s.S + 2),
+                             hipMemcpyDeviceToHost));
+          s.ms_box[0] = s.ms_seq;
+        }
+        break;
+      }
+      std::this_thread::yield();
+    }
+    const int t = take_counts();
+    if (t == 0 || (t == 1 && !*all_ended)) {
+      // the committed offsets (unchanged without an operation)
+      const int *o = h + 4 * s.S + 1;
+      s.poff_h.assign(o, o + s.S + 1);
+      s.np = s.poff_h[s.S];
+      return AMSC_OK;
+    }
+    if (t == 1) return AMSC_OK;  // every run had ended: an unchanged copy
+    // overflow: nothing was executed or committed; back to the current
+    // buffer and its binning
+    s.cur = 1 - s.cur;
+    if (int rc = rebin(s, false)) return rc;
+    return t;
+  }
   if (!capture) {
     // the committed offsets, read back above
-    const int *h = s.host_ints + 4 * s.S + 1;
-    s.poff_h.assign(h, h + s.S + 1);
+    const int *o = s.host_ints + 4 * s.S + 1;
+    s.poff_h.assign(o, o + s.S + 1);
     s.np = s.poff_h[s.S];
   }
   // the execute kernel took the new particles' keys and slots
@@ -975,9 +1058,10 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   for (int **q : ints)
     if (int rc = dalloc(*s, q, cap)) return rc;
   if (int rc = dalloc(*s, &s->offs, cap + 1)) return rc;
-  // operation counts, overflow flag and new offsets of a merge/split step
-  // in one block, read back with one copy
-  if (int rc = dalloc(*s, &s->ms_cnt, 5 * S + 2)) return rc;
+  // operation counts (4 per scenario), overflow flag, new offsets (S + 1)
+  // and active flags (S) of a merge/split step in one block, read back with
+  // one copy
+  if (int rc = dalloc(*s, &s->ms_cnt, 6 * S + 2)) return rc;
   s->err = s->ms_cnt + 4 * S;
   s->poff_next = s->err + 1;
   if (int rc = dalloc(*s, &s->ms_cand, std::max(s->nc, 1))) return rc;
@@ -1007,7 +1091,7 @@ int setup(Sim *s, int S, const amsc_params *prm, const int *np_arr,
   if (int rc = pinned_take(sizeof(Ctrl) * S, (void **)&s->ctrl_host,
                            &s->ctrl_host_bytes))
     return rc;
-  if (int rc = pinned_take(sizeof(int) * (5 * S + 8), (void **)&s->host_ints,
+  if (int rc = pinned_take(sizeof(int) * (6 * S + 8), (void **)&s->host_ints,
                            &s->host_ints_bytes))
     return rc;
   // binning arrays and scan workspaces from the device cache

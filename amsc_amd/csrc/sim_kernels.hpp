@@ -2118,13 +2118,27 @@ __global__ void weights_kernel(Tables T, int n, const int *__restrict__ act,
 }
 
 // New particle offsets of every scenario (order is preserved, so scenario
-// s starts at the scan value of its first particle); flags overflow.
+// s starts at the scan value of its first particle); flags overflow; copies
+// each scenario's active flag next to them (read back with them).  box !=
+// null (host memory): the whole block cnt[0 .. 6S+2) is also written to
+// box[1 ..], then box[0] = seq, released to the host.
 __global__ void ms_offsets_kernel(const int *__restrict__ poff, int S,
                                   const int *__restrict__ offs, int cap,
-                                  int *poff_next, int *err) {
-  for (int s = threadIdx.x; s <= S; s += blockDim.x)
+                                  int *poff_next, int *err,
+                                  const Ctrl *__restrict__ ctrl, int *active,
+                                  int *box, const int *cnt, int seq) {
+  for (int s = threadIdx.x; s <= S; s += blockDim.x) {
     poff_next[s] = offs[poff[s]];
+    if (s < S) active[s] = ctrl[s].active;
+  }
   if (threadIdx.x == 0 && offs[poff[S]] > cap) *err = 1;
+  if (!box) return;
+  __syncthreads();
+  for (int k = threadIdx.x; k < 6 * S + 2; k += blockDim.x) box[1 + k] = cnt[k];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&box[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ void ms_commit_kernel(int *poff, const int *__restrict__ poff_next,

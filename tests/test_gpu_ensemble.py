@@ -304,9 +304,15 @@ def test_capacity_overflow_is_reported():
              ms_hp_min=0.0)
     c = scen(d)
     e = Ensemble([c], "fp64", capacity=len(c[1]["x"]) + 3)
+    n0 = e.num_particles
     with pytest.raises(native.AmscError) as ei:
-        e.step(4)
+        for _ in range(4):
+            x_prev = e.get(0, "x")
+            e.step(1)
     assert ei.value.code == 4
+    # nothing was executed: the particles are those before the failing step
+    assert e.num_particles == n0
+    assert np.array_equal(e.get(0, "x"), x_prev)
     e.close()
 
 
