@@ -1126,11 +1126,12 @@ __device__ __forceinline__ void p2g_sweep_body(P2GArgs a) {
       // CSR starts of step k+3
       P2GRaw<R> WB[2];
       SweepIds IC;
-      // the loads are issued at a raised wave priority: a wavefront that
-      // reaches them is not held behind the other wavefront's fold on the
-      // SIMD, so they go out as early as they can (FP64 step 0.3582 ->
-      // 0.3555 ms on the 4M workload over 200 steps)
-      __builtin_amdgcn_s_setprio(2);
+      // FP64: the loads are issued at a raised wave priority, so that a
+      // wavefront that reaches them is not held behind the other
+      // wavefront's fold on the SIMD (step 0.3582 -> 0.3555 ms on the 4M
+      // workload over 200 steps; the mixed kernel, three wavefronts per
+      // SIMD, was slower with it: 137 -> 149 us inside the steps)
+      if constexpr (sizeof(R) == 8) __builtin_amdgcn_s_setprio(2);
       fetch_fields(IB, WB);
       fetch_ids(csC, IC);
       const int csD = load_cs(j + 3);
@@ -1140,7 +1141,7 @@ __device__ __forceinline__ void p2g_sweep_body(P2GArgs a) {
         if (E - B > CAP) p2g_fields<R>(a, ioA, fric_read ? ioA : 0, WO);
         fetch_ovf(csB, ioN, roN);
       }
-      __builtin_amdgcn_s_setprio(0);
+      if constexpr (sizeof(R) == 8) __builtin_amdgcn_s_setprio(0);
       __syncthreads();
       // this lane's range: cells (j, r-1) and (j, r) where they exist
       int lo = 0, mid = 0, hi = 0;
