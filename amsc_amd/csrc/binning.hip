@@ -41,15 +41,34 @@ __global__ void key_count_kernel(int np, const double *__restrict__ x,
   }
 }
 
+// kPlaceItems particles per thread, a workgroup's width apart: every load
+// of a thread is issued before the first cell start it needs, so the two
+// dependent round trips (key and slot, then the cell start) overlap across
+// its particles (4M workload: 15.9 -> 13.9 us with two).
+constexpr int kPlaceItems = 2;
 __global__ void place_kernel(int np, const int *__restrict__ np_dev,
                              const int *__restrict__ key,
                              const int *__restrict__ slot,
                              const int *__restrict__ cstart, int *cptcls,
                              const int *__restrict__ active) {
   if (active && !*active) return;
-  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= (np_dev ? *np_dev : np)) return;
-  cptcls[cstart[key[ip]] + slot[ip]] = ip;
+  const int n = np_dev ? *np_dev : np;
+  const int i0 = blockIdx.x * (kPlaceItems * blockDim.x) + threadIdx.x;
+  int k[kPlaceItems], sl[kPlaceItems], c[kPlaceItems];
+#pragma unroll
+  for (int j = 0; j < kPlaceItems; ++j) {
+    const int ip = i0 + j * blockDim.x;
+    k[j] = ip < n ? key[ip] : 0;
+    sl[j] = ip < n ? slot[ip] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kPlaceItems; ++j)
+    c[j] = i0 + j * (int)blockDim.x < n ? cstart[k[j]] : 0;
+#pragma unroll
+  for (int j = 0; j < kPlaceItems; ++j) {
+    const int ip = i0 + j * blockDim.x;
+    if (ip < n) cptcls[c[j] + sl[j]] = ip;
+  }
 }
 
 // Heap sort of a[0..n) in place (cells above 8 particles: O(n log n)
@@ -350,7 +369,7 @@ int launch_finish_binning(Binning &b, int np, const int *key,
   if (int rc = scan_launch(b.count, b.cstart, nc, b.scan, st, b.count, b.big))
     return rc;
   if (np > 0)
-    place_kernel<<<grid_for(np, 256), 256, 0, st>>>(np, np_dev, key, slot,
+    place_kernel<<<grid_for(np, kPlaceItems * 256), 256, 0, st>>>(np, np_dev, key, slot,
                                                     b.cstart, cptcls, active);
   cell_sort_kernel<<<grid_for(nc, 256), 256, 0, st>>>(nc, b.cstart, cptcls,
                                                       active, b.big, b.big_cap);
